@@ -1,0 +1,86 @@
+"""Loader for libcfs_plume.so (the C ABI declared in include/cfs_plume.h).
+
+The library is built in-tree by cfs_plume_amd/build.py.  There is no fallback:
+if the library is missing or cannot be loaded, every call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcfs_plume.so")
+
+CFS_OK = 0
+STATUS_NAMES = {
+    -1: "CFS_ERR_ARG", -2: "CFS_ERR_IO", -3: "CFS_ERR_FORMAT", -4: "CFS_ERR_HIP",
+    -5: "CFS_ERR_STREAM", -6: "CFS_ERR_NOMEM",
+}
+
+# Every symbol declared in include/cfs_plume.h, with (restype, argtypes).
+_vp, _i, _sz, _cp = ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t, ctypes.c_char_p
+SIGNATURES = {
+    "cfs_last_error": (ctypes.c_char_p, []),
+    "cfs_abi_version": (_i, []),
+    "cfs_basis_open_files": (_i, [_cp, _cp, _cp, ctypes.POINTER(_vp)]),
+    "cfs_basis_open_memory": (_i, [_vp, _sz, _vp, _sz, _vp, _sz, ctypes.POINTER(_vp)]),
+    "cfs_basis_close": (_i, [_vp]),
+    "cfs_basis_get_info": (_i, [_vp, _vp]),
+    "cfs_project": (_i, [_vp, _vp, _vp, _vp]),
+    "cfs_unproject": (_i, [_vp, _vp, _vp, _vp]),
+    "cfs_step": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "cfs_decode_columns": (_i, [_vp, _i, _i, _vp, _vp]),
+    "cfs_decode_block_ints": (_i, [_vp, _i, _i, _i, _i, _vp, _vp]),
+    "cfs_decode_block_coeffs": (_i, [_vp, _i, _i, _i, _i, _vp, _vp]),
+    "cfs_get_submatrix": (_i, [_vp, _i, _vp, _vp]),
+    "cfs_check_stream_errors": (_i, [_vp, ctypes.POINTER(ctypes.c_uint32)]),
+    "cfs_encode_matrix": (_i, [_vp, _i, _i, _i, _i, _i, ctypes.c_double, _i, _i, ctypes.POINTER(_vp)]),
+    "cfs_encoded_buffer": (_i, [_vp, _i, ctypes.POINTER(_vp), ctypes.POINTER(_sz)]),
+    "cfs_encoded_write": (_i, [_vp, _cp]),
+    "cfs_encoded_free": (None, [_vp]),
+}
+
+
+class CfsError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {message}")
+        self.status = status
+
+
+class BasisInfo(ctypes.Structure):
+    _fields_ = [
+        ("n_bits", ctypes.c_int32),
+        ("dims", ctypes.c_int32 * 3),
+        ("padded", ctypes.c_int32 * 3),
+        ("num_cols", ctypes.c_int32),
+        ("num_blocks", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("tokens", ctypes.c_int64 * 3),
+        ("device_bytes", ctypes.c_int64),
+    ]
+
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """The loaded library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is missing: build it with `python -m cfs_plume_amd.build` "
+                "(the HIP path has no CPU fallback)")
+        h = ctypes.CDLL(LIB_PATH, mode=ctypes.RTLD_GLOBAL)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(h, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = h
+    return _lib
+
+
+def check(status: int, what: str = ""):
+    if status != CFS_OK:
+        msg = lib().cfs_last_error().decode(errors="replace")
+        raise CfsError(status, f"{what}: {msg}" if what else msg)

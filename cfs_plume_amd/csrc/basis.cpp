@@ -1,0 +1,491 @@
+// basis.cpp -- host side of the C ABI (include/cfs_plume.h): component-file
+// parsing and validation, block-major repacking, upload, kernel dispatch.
+//
+// Parsing follows ReadBinaryFileToMemory (reference COMPRESSION.cpp:1456-1594):
+// header int32[10] = version, nBits, X, Y, Z, Px, Py, Pz, r, nB; then sList
+// (f64), gammaList (f64), blockLengths (i32), blockIndices (i32), each nB x r
+// column-major (entry (b, c) at b + nB*c); then int32 tokens to the end of the
+// file.  Where the reference asserts or silently reads out of range, the
+// loader rejects the file with CFS_ERR_FORMAT.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <memory>
+#include <numeric>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/cfs_plume.h"
+#include "cfs_device.h"
+
+namespace cfs {
+// kernels (step_kernels.hip, decode_kernels.hip)
+hipError_t launch_step(const DevBasis& B, int mode, const float* v_in, const double* q_in, double* q_out,
+                       float* field_out, const float* d_damp, const void* d_tab, int grid, hipStream_t s);
+hipError_t step_occupancy(int mode, int r, int nk, int* wg_per_cu);
+size_t tables_bytes();
+void fill_tables(void* host_tab, const float* dct, const uint16_t* zig, const uint16_t* rev, const uint8_t* dz);
+hipError_t launch_decode(const DevBasis& B, int mode, const DecodeArgs& a, const float* d_damp, const void* d_tab,
+                         hipStream_t s);
+}  // namespace cfs
+
+using namespace cfs;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+int hip_fail(hipError_t e, const char* what) {
+  return fail(CFS_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+int pad_of(int n) { return (BS - (n % BS)) % BS; }
+
+struct FileView {
+  int32_t nBits, X, Y, Z, Px, Py, Pz, r, nB;
+  const double* s;
+  const double* gamma;
+  const int32_t* len;
+  const int32_t* idx;
+  const int32_t* tok;
+  size_t ntok;
+};
+
+int parse_file(const uint8_t* d, size_t n, int comp, FileView* f) {
+  const std::string who = "component " + std::to_string(comp) + ": ";
+  if (n < 40) return fail(CFS_ERR_FORMAT, who + "truncated header");
+  int32_t h[10];
+  std::memcpy(h, d, 40);
+  if (h[0] != 1)
+    return fail(CFS_ERR_FORMAT, who + "unsupported version " + std::to_string(h[0]) +
+                                    " (version 0 files store paddings in place of padded dims)");
+  f->nBits = h[1];
+  f->X = h[2]; f->Y = h[3]; f->Z = h[4];
+  f->Px = h[5]; f->Py = h[6]; f->Pz = h[7];
+  f->r = h[8]; f->nB = h[9];
+  if (f->nBits < 1 || f->nBits > 32) return fail(CFS_ERR_FORMAT, who + "nBits out of range");
+  if (f->X < 1 || f->Y < 1 || f->Z < 1) return fail(CFS_ERR_FORMAT, who + "bad dims");
+  if (f->Px != f->X + pad_of(f->X) || f->Py != f->Y + pad_of(f->Y) || f->Pz != f->Z + pad_of(f->Z))
+    return fail(CFS_ERR_FORMAT, who + "padded dims do not match dims");
+  if ((long long)f->nB != (long long)(f->Px / BS) * (f->Py / BS) * (f->Pz / BS))
+    return fail(CFS_ERR_FORMAT, who + "numBlocks does not match padded dims");
+  if (f->r < 1) return fail(CFS_ERR_FORMAT, who + "numCols <= 0");
+  const size_t m = (size_t)f->r * f->nB;
+  const size_t meta = 40 + m * 24;
+  if (n < meta || (n - meta) % 4 != 0) return fail(CFS_ERR_FORMAT, who + "truncated metadata or token stream");
+  // the arrays are 8- or 4-byte aligned inside the image because the header is 40 bytes
+  f->s = reinterpret_cast<const double*>(d + 40);
+  f->gamma = f->s + m;
+  f->len = reinterpret_cast<const int32_t*>(f->gamma + m);
+  f->idx = f->len + m;
+  f->tok = f->idx + m;
+  f->ntok = (n - meta) / 4;
+  for (size_t i = 0; i < m; i++) {
+    const int32_t L = f->len[i], o = f->idx[i];
+    if (L < 1 || L > MAX_SEG_TOKENS) return fail(CFS_ERR_FORMAT, who + "segment length outside [1, 1024]");
+    if (o < 0 || (size_t)o + (size_t)L > f->ntok) return fail(CFS_ERR_FORMAT, who + "segment past end of stream");
+    const double g4 = 4.0 * f->gamma[i];
+    if (!(g4 >= 0.0) || g4 > 4.0 * f->nBits || g4 != std::floor(g4))
+      return fail(CFS_ERR_FORMAT, who + "gamma is not a quarter-integer in [0, nBits]");
+    if (std::isnan(f->s[i]) || f->s[i] == 0.0) return fail(CFS_ERR_FORMAT, who + "scale s is NaN or zero");
+  }
+  return CFS_OK;
+}
+
+void parallel_for(size_t n, const std::function<void(size_t, size_t)>& fn) {
+  unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  if (n < 4096) nt = 1;
+  std::vector<std::thread> th;
+  const size_t per = (n + nt - 1) / nt;
+  for (unsigned t = 0; t < nt; t++) {
+    const size_t a = t * per, b = std::min(n, a + per);
+    if (a >= b) break;
+    th.emplace_back([&fn, a, b] { fn(a, b); });
+  }
+  for (auto& x : th) x.join();
+}
+
+template <class T>
+int dev_upload(T** dst, const T* src, size_t count, size_t* total) {
+  const size_t bytes = std::max<size_t>(count * sizeof(T), 4);
+  hipError_t e = hipMalloc(reinterpret_cast<void**>(dst), bytes);
+  if (e != hipSuccess) return hip_fail(e, "hipMalloc");
+  *total += bytes;
+  if (src && count) {
+    e = hipMemcpy(*dst, src, count * sizeof(T), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_fail(e, "hipMemcpy");
+  }
+  return CFS_OK;
+}
+
+// Host copies of the tables every basis uses.
+struct HostTables {
+  uint16_t zig[BC], rev[BC];
+  uint8_t dz[BC];
+  float dct[64];
+  HostTables() {
+    int j = 0;
+    for (int sum = 0; sum < 3 * BS; sum++)
+      for (int z = 0; z < BS; z++)
+        for (int y = 0; y < BS; y++)
+          for (int x = 0; x < BS; x++)
+            if (x + y + z == sum) zig[x + BS * y + BS * BS * z] = (uint16_t)j++;
+    for (int i = 0; i < BC; i++) rev[zig[i]] = (uint16_t)i;
+    for (int j2 = 0; j2 < BC; j2++) {
+      const int i = rev[j2];
+      dz[j2] = (uint8_t)((i & 7) + ((i >> 3) & 7) + (i >> 6));
+    }
+    for (int k = 0; k < BS; k++)
+      for (int n = 0; n < BS; n++) {
+        const double a = (k == 0) ? std::sqrt(0.125) : 0.5;
+        dct[k * BS + n] = (float)(a * std::cos((M_PI * (double)((2 * n + 1) * k)) / 16.0));
+      }
+  }
+};
+const HostTables& host_tables() {
+  static HostTables t;
+  return t;
+}
+
+}  // namespace
+
+struct cfs_basis {
+  int device = 0;
+  int nBits = 0, X = 0, Y = 0, Z = 0, Px = 0, Py = 0, Pz = 0, r = 0, nB = 0, nk = 0;
+  int64_t ntok[3] = {0, 0, 0};
+  int grid[4] = {0, 0, 0, 0};  // persistent grid per mode (index = mode)
+  size_t device_bytes = 0;
+  DevBasis dev{};
+  float* d_damp = nullptr;
+  void* d_tab = nullptr;
+  std::vector<void*> allocs;
+  ~cfs_basis() {
+    for (void* p : allocs) (void)hipFree(p);
+  }
+};
+
+namespace {
+
+int build_basis(const uint8_t* const bufs[3], const size_t sizes[3], cfs_basis** out) {
+  if (!out) return fail(CFS_ERR_ARG, "null output handle");
+  *out = nullptr;
+  FileView f[3];
+  for (int k = 0; k < 3; k++) {
+    if (!bufs[k]) return fail(CFS_ERR_ARG, "null component buffer");
+    int st = parse_file(bufs[k], sizes[k], k, &f[k]);
+    if (st) return st;
+  }
+  for (int k = 1; k < 3; k++)
+    if (f[k].nBits != f[0].nBits || f[k].X != f[0].X || f[k].Y != f[0].Y || f[k].Z != f[0].Z || f[k].r != f[0].r ||
+        f[k].nB != f[0].nB)
+      return fail(CFS_ERR_FORMAT, "component headers disagree");
+
+  std::unique_ptr<cfs_basis> B(new cfs_basis);
+  hipError_t e = hipGetDevice(&B->device);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+  B->nBits = f[0].nBits;
+  B->X = f[0].X; B->Y = f[0].Y; B->Z = f[0].Z;
+  B->Px = f[0].Px; B->Py = f[0].Py; B->Pz = f[0].Pz;
+  B->r = f[0].r; B->nB = f[0].nB;
+  B->nk = 4 * B->nBits + 1;
+  const int r = B->r, nB = B->nB;
+  const size_t nseg = (size_t)3 * nB * r;
+
+  // block-major layout: stream (k, b) holds segments (k, b, 0..r-1)
+  std::vector<uint64_t> blk_base((size_t)3 * nB);
+  std::vector<uint32_t> blk_len((size_t)3 * nB);
+  std::vector<uint32_t> seg_lk(nseg), seg_off(nseg);
+  std::vector<float> seg_sinv(nseg);
+  uint64_t pos = 0;
+  for (int k = 0; k < 3; k++) {
+    B->ntok[k] = (int64_t)f[k].ntok;
+    for (int b = 0; b < nB; b++) {
+      const size_t kb = (size_t)k * nB + b;
+      blk_base[kb] = pos;
+      uint64_t off = 0;
+      for (int c = 0; c < r; c++) {
+        const size_t e2 = (size_t)b + (size_t)nB * c;  // file order
+        const size_t si = kb * r + c;                   // device order
+        const uint32_t L = (uint32_t)f[k].len[e2];
+        const uint32_t k4 = (uint32_t)(4.0 * f[k].gamma[e2]);
+        seg_lk[si] = L | (k4 << 16);
+        seg_off[si] = (uint32_t)off;
+        seg_sinv[si] = (float)(std::ldexp(1.0 / f[k].s[e2], FACTOR_SHIFT));
+        off += L;
+      }
+      if (off > 0xffffffffull) return fail(CFS_ERR_FORMAT, "block stream too long");
+      blk_len[kb] = (uint32_t)off;
+      pos += (off + 3) & ~(uint64_t)3;
+    }
+  }
+  const uint64_t total_tok = pos + GUARD_TOKENS;
+  std::vector<int32_t> tokens;
+  try {
+    tokens.assign(total_tok, 0);
+  } catch (...) {
+    return fail(CFS_ERR_NOMEM, "host allocation for repacked tokens failed");
+  }
+  parallel_for((size_t)3 * nB, [&](size_t a, size_t bnd) {
+    for (size_t kb = a; kb < bnd; kb++) {
+      const int k = (int)(kb / nB), b = (int)(kb % nB);
+      int32_t* dst = tokens.data() + blk_base[kb];
+      for (int c = 0; c < r; c++) {
+        const size_t e2 = (size_t)b + (size_t)nB * c;
+        const int32_t L = f[k].len[e2];
+        std::memcpy(dst, f[k].tok + f[k].idx[e2], (size_t)L * 4);
+        dst += L;
+      }
+    }
+  });
+  // work order: blocks with the longest streams first
+  std::vector<int32_t> order(nB);
+  std::iota(order.begin(), order.end(), 0);
+  std::vector<uint64_t> work(nB);
+  for (int b = 0; b < nB; b++) work[b] = (uint64_t)blk_len[b] + blk_len[(size_t)nB + b] + blk_len[(size_t)2 * nB + b];
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return work[a] > work[b]; });
+
+  // damping table (1+d)^(k/4), FP64 -> FP32 (COMPRESSION_DATA.cpp:25-43),
+  // scaled by 2^-FACTOR_SHIFT (and 1/s by 2^FACTOR_SHIFT) so that 22^32 fits FP32
+  std::vector<float> damp((size_t)B->nk * NDIAG);
+  for (int k = 0; k < B->nk; k++)
+    for (int d = 0; d < NDIAG; d++)
+      damp[(size_t)k * NDIAG + d] = (float)std::ldexp(std::pow((double)(1 + d), k / 4.0), -FACTOR_SHIFT);
+  std::vector<uint8_t> tab(tables_bytes());
+  const HostTables& T = host_tables();
+  fill_tables(tab.data(), T.dct, T.zig, T.rev, T.dz);
+
+  // persistent grid sizes
+  int cus = 0;
+  e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, B->device);
+  if (e != hipSuccess) return hip_fail(e, "hipDeviceGetAttribute");
+  int maxgrid = 1;
+  for (int mode = 1; mode <= 3; mode++) {
+    int per = 0;
+    e = step_occupancy(mode, r, B->nk, &per);
+    if (e != hipSuccess) return hip_fail(e, "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+    if (per < 1) return fail(CFS_ERR_ARG, "numCols too large for the step kernel's LDS budget");
+    B->grid[mode] = std::max(1, std::min(cus * per, 3 * nB));
+    maxgrid = std::max(maxgrid, B->grid[mode]);
+  }
+
+  size_t tot = 0;
+  DevBasis& D = B->dev;
+  int32_t* d_tok = nullptr;
+  uint64_t* d_bb = nullptr;
+  uint32_t *d_bl = nullptr, *d_lk = nullptr, *d_off = nullptr, *d_err = nullptr, *d_cnt = nullptr;
+  float *d_sinv = nullptr, *d_damp = nullptr;
+  int32_t* d_order = nullptr;
+  double* d_part = nullptr;
+  uint8_t* d_tab = nullptr;
+  int st = CFS_OK;
+#define UP(ptr, src, cnt)                        \
+  do {                                           \
+    st = dev_upload(&ptr, src, cnt, &tot);       \
+    if (ptr) B->allocs.push_back((void*)ptr);    \
+    if (st) return st;                           \
+  } while (0)
+  UP(d_tok, tokens.data(), tokens.size());
+  UP(d_bb, blk_base.data(), blk_base.size());
+  UP(d_bl, blk_len.data(), blk_len.size());
+  UP(d_lk, seg_lk.data(), seg_lk.size());
+  UP(d_off, seg_off.data(), seg_off.size());
+  UP(d_sinv, seg_sinv.data(), seg_sinv.size());
+  UP(d_damp, damp.data(), damp.size());
+  UP(d_tab, tab.data(), tab.size());
+  UP(d_order, order.data(), order.size());
+  const uint32_t zeros[4] = {0, 0, 0, 0};
+  UP(d_err, zeros, 1);
+  UP(d_cnt, zeros, 2);
+  UP(d_part, (const double*)nullptr, (size_t)maxgrid * r);
+#undef UP
+  D.tokens = d_tok;
+  D.blk_base = d_bb;
+  D.blk_len = d_bl;
+  D.seg_lk = d_lk;
+  D.seg_off = d_off;
+  D.seg_sinv = d_sinv;
+  D.damp = d_damp;
+  D.order = d_order;
+  D.err = d_err;
+  D.counter = d_cnt;
+  D.partials = d_part;
+  D.X = B->X; D.Y = B->Y; D.Z = B->Z;
+  D.nbx = B->Px / BS; D.nby = B->Py / BS; D.nbz = B->Pz / BS;
+  D.nB = nB; D.r = r; D.nk = B->nk;
+  B->d_damp = d_damp;
+  B->d_tab = d_tab;
+  B->device_bytes = tot;
+  *out = B.release();
+  return CFS_OK;
+}
+
+struct DeviceGuard {
+  int prev = -1;
+  explicit DeviceGuard(int dev) {
+    if (hipGetDevice(&prev) == hipSuccess && prev != dev) (void)hipSetDevice(dev);
+    else prev = -1;
+  }
+  ~DeviceGuard() {
+    if (prev >= 0) (void)hipSetDevice(prev);
+  }
+};
+
+int read_file(const char* path, std::vector<uint8_t>* buf) {
+  if (!path) return fail(CFS_ERR_ARG, "null path");
+  FILE* fp = std::fopen(path, "rb");
+  if (!fp) return fail(CFS_ERR_IO, std::string("cannot open ") + path);
+  std::fseek(fp, 0, SEEK_END);
+  const long n = std::ftell(fp);
+  std::fseek(fp, 0, SEEK_SET);
+  if (n < 0) {
+    std::fclose(fp);
+    return fail(CFS_ERR_IO, std::string("cannot size ") + path);
+  }
+  buf->resize((size_t)n);
+  const size_t got = n ? std::fread(buf->data(), 1, (size_t)n, fp) : 0;
+  std::fclose(fp);
+  if (got != (size_t)n) return fail(CFS_ERR_IO, std::string("short read on ") + path);
+  return CFS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* cfs_last_error(void) { return g_err.c_str(); }
+int cfs_abi_version(void) { return CFS_ABI_VERSION; }
+
+int cfs_basis_open_memory(const void* bx, size_t nx, const void* by, size_t ny, const void* bz, size_t nz,
+                          cfs_basis** out) {
+  g_err.clear();
+  const uint8_t* bufs[3] = {(const uint8_t*)bx, (const uint8_t*)by, (const uint8_t*)bz};
+  const size_t sizes[3] = {nx, ny, nz};
+  return build_basis(bufs, sizes, out);
+}
+
+int cfs_basis_open_files(const char* px, const char* py, const char* pz, cfs_basis** out) {
+  g_err.clear();
+  std::vector<uint8_t> b[3];
+  const char* p[3] = {px, py, pz};
+  for (int k = 0; k < 3; k++) {
+    int st = read_file(p[k], &b[k]);
+    if (st) return st;
+  }
+  const uint8_t* bufs[3] = {b[0].data(), b[1].data(), b[2].data()};
+  const size_t sizes[3] = {b[0].size(), b[1].size(), b[2].size()};
+  return build_basis(bufs, sizes, out);
+}
+
+int cfs_basis_close(cfs_basis* basis) {
+  if (!basis) return fail(CFS_ERR_ARG, "null basis");
+  DeviceGuard g(basis->device);
+  delete basis;
+  return CFS_OK;
+}
+
+int cfs_basis_get_info(const cfs_basis* B, cfs_basis_info* info) {
+  if (!B || !info) return fail(CFS_ERR_ARG, "null argument");
+  info->n_bits = B->nBits;
+  info->dims[0] = B->X; info->dims[1] = B->Y; info->dims[2] = B->Z;
+  info->padded[0] = B->Px; info->padded[1] = B->Py; info->padded[2] = B->Pz;
+  info->num_cols = B->r;
+  info->num_blocks = B->nB;
+  info->reserved = 0;
+  for (int k = 0; k < 3; k++) info->tokens[k] = B->ntok[k];
+  info->device_bytes = (int64_t)B->device_bytes;
+  return CFS_OK;
+}
+
+int cfs_project(cfs_basis* B, const float* v, double* q, void* stream) {
+  if (!B || !v || !q) return fail(CFS_ERR_ARG, "null argument");
+  DeviceGuard g(B->device);
+  hipError_t e = launch_step(B->dev, 1, v, nullptr, q, nullptr, B->d_damp, B->d_tab, B->grid[1], (hipStream_t)stream);
+  return e == hipSuccess ? CFS_OK : hip_fail(e, "project launch");
+}
+
+int cfs_unproject(cfs_basis* B, const double* q, float* field, void* stream) {
+  if (!B || !q || !field) return fail(CFS_ERR_ARG, "null argument");
+  DeviceGuard g(B->device);
+  hipError_t e =
+      launch_step(B->dev, 2, nullptr, q, nullptr, field, B->d_damp, B->d_tab, B->grid[2], (hipStream_t)stream);
+  return e == hipSuccess ? CFS_OK : hip_fail(e, "unproject launch");
+}
+
+int cfs_step(cfs_basis* pre, cfs_basis* fin, const float* v_in, double* q_out, const double* q_in, float* field,
+             void* stream) {
+  if (!pre || !fin || !v_in || !q_out || !q_in || !field) return fail(CFS_ERR_ARG, "null argument");
+  if (pre == fin) {
+    DeviceGuard g(pre->device);
+    hipError_t e = launch_step(pre->dev, 3, v_in, q_in, q_out, field, pre->d_damp, pre->d_tab, pre->grid[3],
+                               (hipStream_t)stream);
+    return e == hipSuccess ? CFS_OK : hip_fail(e, "step launch");
+  }
+  int st = cfs_project(pre, v_in, q_out, stream);
+  if (st) return st;
+  return cfs_unproject(fin, q_in, field, stream);
+}
+
+static int decode_common(cfs_basis* B, int mode, int k0, int nk3, int col0, int ncol, int b0, int nblk, int point,
+                         void* out, void* stream) {
+  if (!B || !out) return fail(CFS_ERR_ARG, "null argument");
+  if (k0 < 0 || k0 + nk3 > 3 || nk3 < 1) return fail(CFS_ERR_ARG, "component out of range");
+  if (col0 < 0 || ncol < 0 || col0 + ncol > B->r) return fail(CFS_ERR_ARG, "column range out of bounds");
+  if (b0 < 0 || nblk < 0 || b0 + nblk > B->nB) return fail(CFS_ERR_ARG, "block range out of bounds");
+  DeviceGuard g(B->device);
+  DecodeArgs a{k0, nk3, col0, ncol, b0, nblk, point, out};
+  hipError_t e = launch_decode(B->dev, mode, a, B->d_damp, B->d_tab, (hipStream_t)stream);
+  return e == hipSuccess ? CFS_OK : hip_fail(e, "decode launch");
+}
+
+int cfs_decode_block_ints(cfs_basis* B, int comp, int col, int block0, int nblocks, int32_t* out, void* stream) {
+  return decode_common(B, 0, comp, 1, col, 1, block0, nblocks, 0, out, stream);
+}
+int cfs_decode_block_coeffs(cfs_basis* B, int comp, int col, int block0, int nblocks, float* out, void* stream) {
+  return decode_common(B, 1, comp, 1, col, 1, block0, nblocks, 0, out, stream);
+}
+int cfs_decode_columns(cfs_basis* B, int col0, int ncols, float* out, void* stream) {
+  if (!B) return fail(CFS_ERR_ARG, "null basis");
+  return decode_common(B, 2, 0, 3, col0, ncols, 0, B->nB, 0, out, stream);
+}
+int cfs_get_submatrix(cfs_basis* B, int start_row, double* out, void* stream) {
+  if (!B) return fail(CFS_ERR_ARG, "null basis");
+  const long long rows = 3LL * B->X * B->Y * B->Z;
+  if (start_row < 0 || start_row % 3 != 0 || start_row >= rows)
+    return fail(CFS_ERR_ARG, "start_row must be a multiple of 3 inside the matrix");
+  // ComputeBlockNumber (COMPRESSION.cpp:1712-1749)
+  const int index = start_row / 3;
+  const int z = index / (B->X * B->Y);
+  const int rem = index - B->X * B->Y * z;
+  const int y = rem / B->X;
+  const int x = rem - B->X * y;
+  const int point = (x % BS) + BS * (y % BS) + BS * BS * (z % BS);
+  const int nbx = B->Px / BS, nby = B->Py / BS;
+  const int blk = x / BS + (y / BS) * nbx + (z / BS) * nbx * nby;
+  return decode_common(B, 3, 0, 3, 0, B->r, blk, 1, point, out, stream);
+}
+
+int cfs_check_stream_errors(cfs_basis* B, uint32_t* bits) {
+  if (!B || !bits) return fail(CFS_ERR_ARG, "null argument");
+  DeviceGuard g(B->device);
+  hipError_t e = hipDeviceSynchronize();
+  if (e != hipSuccess) return hip_fail(e, "hipDeviceSynchronize");
+  uint32_t v = 0;
+  e = hipMemcpy(&v, B->dev.err, 4, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return hip_fail(e, "hipMemcpy");
+  e = hipMemset(B->dev.err, 0, 4);
+  if (e != hipSuccess) return hip_fail(e, "hipMemset");
+  *bits = v;
+  return CFS_OK;
+}
+
+}  // extern "C"

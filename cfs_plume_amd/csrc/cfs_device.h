@@ -1,0 +1,67 @@
+// cfs_device.h -- device-side layout of a compressed basis on the GPU.
+//
+// A basis (three component files in the reference format v1,
+// COMPRESSION.cpp:2165-2253) is repacked once at load time:
+//
+//   * Tokens are regrouped "block-major": for component k and block b, the r
+//     segments (k, b, c=0..r-1) are stored back to back, forming one "block
+//     stream".  The file stores them column-major (all blocks of column 0,
+//     then column 1, ...), which scatters a block's segments across the file.
+//     Each block stream starts on a 16-byte boundary; the array ends with a
+//     guard of zero tokens so full-width loads past a stream stay in bounds.
+//   * Per segment, in the same block-major order: seg_lk = length | (4*gamma)
+//     << 16 and seg_sinv = (float)(1/s); seg_off = offset inside the stream.
+//   * The damping table (1+d)^(k/4) for d = u+v+w in [0,21] and k = 4*gamma in
+//     [0, 4*nBits] (COMPRESSION_DATA.cpp:25-43 restricted to the 22 distinct
+//     bases) is rounded once from FP64 to FP32.  Table entries are stored times
+//     2^-FACTOR_SHIFT and 1/s times 2^FACTOR_SHIFT: 22^32 = 9.4e42 exceeds the
+//     FP32 range, the scaled pair does not, and their product is unchanged.
+#pragma once
+#include <stdint.h>
+
+namespace cfs {
+
+constexpr int BS = 8;
+constexpr int BC = 512;
+constexpr int NDIAG = 22;      // distinct values of u+v+w
+constexpr int TPL = 4;         // tokens per lane per chunk in the stream decoder
+constexpr int CHUNK = 64 * TPL;
+constexpr int GUARD_TOKENS = 2 * CHUNK;
+constexpr int MAX_SEG_TOKENS = 2 * BC;  // reference assert COMPRESSION.cpp:1206
+constexpr int FACTOR_SHIFT = 64;
+
+// error bits raised by the device decoders
+enum : uint32_t {
+  ERR_RUN_LENGTH = 1u << 0,   // run length outside [2, 512]
+  ERR_SEG_SIZE = 1u << 1,     // a segment does not decode to exactly 512 values
+  ERR_TRUNCATED = 1u << 2,    // a segment ends inside a run
+  ERR_SEG_INDEX = 1u << 3,    // decoded positions run past the last segment
+};
+
+struct DevBasis {
+  const int32_t* tokens;      // block streams + guard
+  const uint64_t* blk_base;   // [3*nB] first token of block stream (k, b)
+  const uint32_t* blk_len;    // [3*nB] tokens in block stream (k, b)
+  const uint32_t* seg_lk;     // [3*nB*r] length | k4 << 16, block-major
+  const uint32_t* seg_off;    // [3*nB*r] offset of segment inside its block stream
+  const float* seg_sinv;      // [3*nB*r] 2^FACTOR_SHIFT / s
+  const float* damp;          // [nk * 22] (1+d)^(k/4) * 2^-FACTOR_SHIFT in FP32
+  const int32_t* order;       // [nB] blocks in descending work order
+  uint32_t* err;              // device error bits
+  uint32_t* counter;          // persistent-kernel work counter (left at 0 after each step)
+  double* partials;           // [grid][r] per-workgroup FP64 partial q
+  int X, Y, Z;                // peeled dims
+  int nbx, nby, nbz, nB;      // blocks per axis, total
+  int r, nk;                  // columns, 4*nBits+1
+};
+
+// arguments of the random-access decode kernel (decode_kernels.hip)
+struct DecodeArgs {
+  int k0, nk3;      // components [k0, k0+nk3)
+  int col0, ncol;   // columns [col0, col0+ncol)
+  int b0, nblk;     // blocks [b0, b0+nblk)
+  int point;        // DEC_POINT: in-block cell x + 8y + 64z
+  void* out;
+};
+
+}  // namespace cfs

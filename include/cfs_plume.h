@@ -1,0 +1,132 @@
+/* cfs_plume.h -- C ABI of the MI355X compressed-subspace step.
+ *
+ * Drop-in boundary for the reference's compressed linear algebra
+ * (aadjones/CFS_PLUME, src/compression/COMPRESSION.h).  Every entry point is
+ * plain C: opaque handles, raw pointers, sizes and int status codes (0 = ok,
+ * < 0 = error; cfs_last_error() gives a message for the calling thread).
+ * Device pointers are HIP device allocations; `stream` is a hipStream_t
+ * (NULL = the default stream).  Calls on one basis handle must not overlap.
+ */
+#ifndef CFS_PLUME_H
+#define CFS_PLUME_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CFS_ABI_VERSION 1
+
+/* status codes */
+#define CFS_OK 0
+#define CFS_ERR_ARG (-1)     /* bad argument (null pointer, range) */
+#define CFS_ERR_IO (-2)      /* file could not be opened / read / written */
+#define CFS_ERR_FORMAT (-3)  /* malformed component file */
+#define CFS_ERR_HIP (-4)     /* HIP runtime error */
+#define CFS_ERR_STREAM (-5)  /* the device decoder found a malformed token stream */
+#define CFS_ERR_NOMEM (-6)   /* host or device allocation failed */
+
+/* device stream-error bits reported by cfs_check_stream_errors */
+#define CFS_STREAM_RUN_LENGTH 0x1u /* run length outside [2, 512] */
+#define CFS_STREAM_SEG_SIZE 0x2u   /* a segment does not decode to 512 values */
+#define CFS_STREAM_TRUNCATED 0x4u  /* a segment ends inside a run */
+#define CFS_STREAM_SEG_INDEX 0x8u  /* decoded positions past the last segment */
+
+typedef struct cfs_basis cfs_basis;
+typedef struct cfs_encoded cfs_encoded;
+
+typedef struct {
+  int32_t n_bits;        /* quantiser bits (COMPRESSION_DATA::get_nBits) */
+  int32_t dims[3];       /* peeled grid X, Y, Z */
+  int32_t padded[3];     /* padded to multiples of 8 */
+  int32_t num_cols;      /* r */
+  int32_t num_blocks;    /* nB per component */
+  int32_t reserved;
+  int64_t tokens[3];     /* int32 tokens per component file */
+  int64_t device_bytes;  /* device memory held by the handle */
+} cfs_basis_info;
+
+/* Message for the last failing call on this thread ("" if none). */
+const char* cfs_last_error(void);
+int cfs_abi_version(void);
+
+/* ---- load -----------------------------------------------------------------
+ * Replaces ReadBinaryFileToMemory x3 + MATRIX_COMPRESSION_DATA construction +
+ * set_dampingArrayLists + dct_setup + init_cache
+ * (COMPRESSION.cpp:1456-1594; MATRIX_COMPRESSION_DATA.h:59-122;
+ *  SUBSPACE_FLUID_3D_COMPRESSED_EIGEN.cpp:1488-1520, 1385-1393).
+ * Reads the three component files (format v1), validates them, regroups the
+ * tokens block-major and uploads them to the current HIP device.  Where the
+ * reference returns NULL and prints, these return CFS_ERR_IO/FORMAT. */
+int cfs_basis_open_files(const char* path_x, const char* path_y, const char* path_z, cfs_basis** out);
+/* Same from three in-memory file images (e.g. cfs_encoded_buffer). */
+int cfs_basis_open_memory(const void* buf_x, size_t size_x, const void* buf_y, size_t size_y, const void* buf_z,
+                          size_t size_z, cfs_basis** out);
+int cfs_basis_close(cfs_basis* basis);
+int cfs_basis_get_info(const cfs_basis* basis, cfs_basis_info* info);
+
+/* ---- per-step operators (device pointers) ---------------------------------
+ * v_full / field: full grid (X+2)(Y+2)(Z+2) cells, x fastest, interleaved
+ * xyz, FP32 (VECTOR3_FIELD_3D layout, VECTOR3_FIELD_3D.cpp:69-85).
+ * q: FP64[r]. */
+
+/* PeeledCompressedProjectTransformNoSVD (COMPRESSION.cpp:2978-3016):
+ * q = U^T v over the peeled cells, evaluated in the DCT domain. */
+int cfs_project(cfs_basis* basis, const float* v_full, double* q, void* stream);
+
+/* PeeledCompressedUnprojectTransform (COMPRESSION.cpp:2739-2900):
+ * interior cells of field = U q; boundary cells are left untouched
+ * (VECTOR3_FIELD_3D::setWithPeeled, VECTOR3_FIELD_3D.cpp:3415-3430). */
+int cfs_unproject(cfs_basis* basis, const double* q, float* field, void* stream);
+
+/* The per-step pair of stepPlume (SUBSPACE_FLUID_3D_COMPRESSED_EIGEN.cpp:477,490):
+ * q_out = U_pre^T v_in and field = U_final q_in.  When pre == fin the two are
+ * fused into one pass that decodes every segment once. */
+int cfs_step(cfs_basis* pre, cfs_basis* fin, const float* v_in, double* q_out, const double* q_in, float* field,
+             void* stream);
+
+/* DecodeVectorField (COMPRESSION.cpp:2640-2670) for columns
+ * [col0, col0+ncols): out holds ncols columns of 3*X*Y*Z FP32 values, peeled
+ * and unpadded, interleaved xyz (the layout of one column of U). */
+int cfs_decode_columns(cfs_basis* basis, int col0, int ncols, float* out, void* stream);
+
+/* RunLengthDecodeBinary + ZigzagUnflatten (COMPRESSION.cpp:1198-1283,
+ * 1427-1445) for blocks [block0, block0+nblocks) of (comp, col): out holds
+ * nblocks x 512 int32 in u+8v+64w order.  Bit-exact check hook. */
+int cfs_decode_block_ints(cfs_basis* basis, int comp, int col, int block0, int nblocks, int32_t* out, void* stream);
+
+/* DecodeBlockWithCompressionData (COMPRESSION.cpp:1003-1057) for the same
+ * range: FP32 coefficients int * (damp^gamma / s), u+8v+64w order. */
+int cfs_decode_block_coeffs(cfs_basis* basis, int comp, int col, int block0, int nblocks, float* out, void* stream);
+
+/* GetSubmatrixNoSVDSparse (COMPRESSION.cpp:1949-2158): the 3 x r rows
+ * [start_row, start_row+3) of U, start_row a multiple of 3; out is FP64
+ * column-major (out[comp + 3*col]) in device memory. */
+int cfs_get_submatrix(cfs_basis* basis, int start_row, double* out, void* stream);
+
+/* Synchronises, returns the CFS_STREAM_* bits raised since the last call and
+ * clears them.  Replaces the reference's asserts (COMPRESSION.cpp:1206,1267). */
+int cfs_check_stream_errors(cfs_basis* basis, uint32_t* bits);
+
+/* ---- encoder (host) ---------------------------------------------------------
+ * CompressAndWriteMatrixComponents (COMPRESSION.cpp:2271-2359) with the SVD
+ * transform off, as on the live path (:2312).  U is FP64 column-major:
+ * column c holds 3*X*Y*Z rows, row 3*(x + X*(y + Y*z)) + comp.  Each block:
+ * edge-replicated padding, orthonormal DCT, scale by (2^(nBits-1)-1)/absMax,
+ * gamma binary search (TuneGammaQuantized :841-916), rint quantisation,
+ * zigzag, RLE.  `threads` <= 0 uses all hardware threads. */
+int cfs_encode_matrix(const double* U, int X, int Y, int Z, int r, int n_bits, double percent, int max_iterations,
+                      int threads, cfs_encoded** out);
+/* The in-memory image of component file `comp` (0, 1, 2). */
+int cfs_encoded_buffer(const cfs_encoded* enc, int comp, const void** data, size_t* size);
+/* Writes <prefix>0, <prefix>1, <prefix>2 (the reference names U.<basis>.component0..2). */
+int cfs_encoded_write(const cfs_encoded* enc, const char* prefix);
+void cfs_encoded_free(cfs_encoded* enc);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CFS_PLUME_H */

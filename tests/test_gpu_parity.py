@@ -1,0 +1,226 @@
+"""GPU parity: the HIP path (through the C ABI) against the FP64 CPU oracle.
+
+Integer stages (RLE tokens -> zigzag -> ints) must match bit for bit.  FP32
+results are compared with relative L2 error against the oracle's FP64:
+  decoded coefficients / columns / submatrix  <= 1e-5
+  projected q, unprojected field              <= 1e-5   (north-star bound)
+"""
+import numpy as np
+import pytest
+
+from cfs_plume_amd import CompressedBasis, DeviceArray, encode_matrix, step, synth
+from oracle import oracle as O
+from tests import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+def rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def _pair(bufs, tmp_path_factory, name):
+    d = tmp_path_factory.mktemp(name)
+    paths = H.write_files(bufs, str(d / "U.test.component"))
+    return O.Basis(paths), CompressedBasis.from_files(paths)
+
+
+@pytest.fixture(scope="module")
+def smooth(tmp_path_factory, gpu):
+    dims = (30, 22, 14)
+    U = synth.synthetic_basis(dims, 8, seed=3)
+    print(f"[case] smooth basis dims={dims} r=8", flush=True)
+    return _pair(encode_matrix(U, dims).buffers(), tmp_path_factory, "smooth")
+
+
+@pytest.fixture(scope="module")
+def rough(tmp_path_factory, gpu):
+    # high wavenumbers on a tiny grid: segments of ~500-770 tokens (several chunks)
+    dims = (14, 14, 6)
+    U = synth.synthetic_basis(dims, 3, seed=5)
+    print(f"[case] rough basis dims={dims} r=3", flush=True)
+    return _pair(encode_matrix(U, dims).buffers(), tmp_path_factory, "rough")
+
+
+def adversarial_zz(k, b, c):
+    """Zigzag int vectors exercising the RLE grammar's corner cases."""
+    rng = np.random.default_rng(1000 * k + 10 * b + c)
+    kind = (b + c + k) % 7
+    z = np.zeros(512, np.int32)
+    if kind == 0:        # all zeros -> [0, 0, 512]
+        pass
+    elif kind == 1:      # no runs: 512 singletons
+        z[:] = np.arange(1, 513) * (1 if c % 2 else -1)
+    elif kind == 2:      # pairs: a, a, b, b, ... -> 768 tokens (the maximum)
+        z[:] = np.repeat(np.arange(1, 257), 2)
+    elif kind == 3:      # run lengths equal to neighbouring values: [5,5,5,k] style
+        z[:3] = 2147483647
+        z[3:8] = 5
+        z[8] = 5 + 1
+        z[9:11] = 2
+        z[11:13] = 2 + 1  # value 3 after a run of length 2 ending in a length token 2
+        z[13:15] = 3
+        z[15:515 - 100] = 0
+        z[412:] = rng.integers(-3, 4, size=100)
+    elif kind == 4:      # long nonzero run then zeros
+        z[0] = -7
+        z[1:300] = 1
+    elif kind == 5:      # random small values with many short zero runs
+        z[:] = rng.integers(-2, 3, size=512)
+    else:                # dense DCT-like decay
+        z[:] = np.round(rng.standard_normal(512) * 2.0e9 / (1 + np.arange(512)) ** 2).astype(np.int64).clip(-2**31 + 1, 2**31 - 1)
+    return z
+
+
+@pytest.fixture(scope="module")
+def adversarial(tmp_path_factory, gpu):
+    dims = (17, 9, 8)  # 3 x 2 x 1 blocks
+    print(f"[case] adversarial token streams dims={dims} r=5", flush=True)
+    bufs = H.basis_from_ints(dims, 5, adversarial_zz, seed=11)
+    return _pair(bufs, tmp_path_factory, "adv")
+
+
+CASES = ["smooth", "rough", "adversarial"]
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_decode_ints_bit_exact(case, request):
+    ob, gb = request.getfixturevalue(case)
+    for k in range(3):
+        for c in range(ob.r):
+            got = gb.decode_block_ints(k, c)
+            want = np.stack([ob.decode_ints(k, b, c) for b in range(ob.nB)])
+            assert np.array_equal(got, want), f"comp {k} col {c}"
+    assert gb.check_stream_errors() == 0
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_decode_coeffs(case, request):
+    ob, gb = request.getfixturevalue(case)
+    for k in range(3):
+        for c in range(ob.r):
+            got = gb.decode_block_coeffs(k, c)
+            want = np.stack([ob.decode_coeffs(k, b, c) for b in range(ob.nB)])
+            assert rel(got, want) <= TOL, (k, c, rel(got, want))
+            # the table path (DecodeBlockWithCompressionDataSparseQuantized) agrees with pow
+            want2 = np.stack([ob.decode_coeffs(k, b, c, table=True) for b in range(ob.nB)])
+            assert rel(want2, want) <= 1e-14
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_decode_columns(case, request):
+    ob, gb = request.getfixturevalue(case)
+    cols = gb.decode_columns().numpy()
+    for c in range(ob.r):
+        want = ob.decode_vector_field(c)
+        assert rel(cols[c], want) <= TOL, (c, rel(cols[c], want))
+
+
+def _fields(ob, seed):
+    v = synth.gaussian_field(ob.dims, seed)
+    q = synth.gaussian_q(ob.r, seed + 1)
+    return v, q
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_project(case, request):
+    ob, gb = request.getfixturevalue(case)
+    v, _ = _fields(ob, 42)
+    want = ob.project(v)
+    dv = DeviceArray.from_numpy(v.astype(np.float32))
+    got = gb.project(dv).numpy()
+    # the oracle sees the same FP32-rounded field
+    want32 = ob.project(v.astype(np.float32).astype(np.float64))
+    assert rel(got, want32) <= TOL, rel(got, want32)
+    assert rel(got, want) <= 1e-4
+    assert gb.check_stream_errors() == 0
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_unproject_keeps_boundary(case, request):
+    ob, gb = request.getfixturevalue(case)
+    _, q = _fields(ob, 43)
+    X, Y, Z = ob.dims
+    init = np.full((Z + 2) * (Y + 2) * (X + 2) * 3, 7.25)
+    want = ob.unproject(q, init)
+    dq = DeviceArray.from_numpy(q)
+    df = DeviceArray.from_numpy(init.astype(np.float32))
+    got = gb.unproject(dq, df).numpy().astype(np.float64)
+    g4 = got.reshape(Z + 2, Y + 2, X + 2, 3)
+    w4 = want.reshape(Z + 2, Y + 2, X + 2, 3)
+    assert rel(g4[1:-1, 1:-1, 1:-1], w4[1:-1, 1:-1, 1:-1]) <= TOL
+    inner = np.zeros_like(g4, bool)
+    inner[1:-1, 1:-1, 1:-1] = True
+    assert np.all(g4[~inner] == 7.25), "unproject wrote boundary cells"
+    assert gb.check_stream_errors() == 0
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_fused_step_matches_separate(case, request):
+    ob, gb = request.getfixturevalue(case)
+    v, q = _fields(ob, 44)
+    dv = DeviceArray.from_numpy(v.astype(np.float32))
+    dq = DeviceArray.from_numpy(q)
+    qo = DeviceArray.zeros((ob.r,), np.float64)
+    fo = DeviceArray.zeros((len(v),), np.float32)
+    results = []
+    for _ in range(3):  # repeated launches: the persistent work counter must reset
+        step(gb, gb, dv, qo, dq, fo)
+        results.append((qo.numpy(), fo.numpy()))
+    q_sep = gb.project(dv).numpy()
+    f_sep = gb.unproject(dq, DeviceArray.zeros((len(v),), np.float32)).numpy()
+    for qq, ff in results:
+        assert rel(qq, q_sep) <= 1e-6
+        assert rel(ff, f_sep) <= 1e-6
+    want_q = ob.project(v.astype(np.float32).astype(np.float64))
+    want_f = ob.unproject(q, np.zeros(len(v)))
+    assert rel(results[-1][0], want_q) <= TOL
+    assert rel(results[-1][1], want_f) <= TOL
+    assert gb.check_stream_errors() == 0
+
+
+@pytest.mark.parametrize("case", ["smooth", "adversarial"])
+def test_submatrix(case, request):
+    ob, gb = request.getfixturevalue(case)
+    X, Y, Z = ob.dims
+    rng = np.random.default_rng(9)
+    cells = [0, 1, X - 1, X * Y * Z - 1] + list(rng.integers(0, X * Y * Z, size=6))
+    for cell in cells:
+        got = gb.get_submatrix(3 * int(cell))
+        want = ob.get_submatrix(3 * int(cell))
+        assert rel(got, want) <= TOL, (cell, rel(got, want))
+
+
+def _bad_basis(tmp_path, mutate):
+    dims = (9, 8, 8)  # 2 blocks
+    zz = [[O.rle_encode(adversarial_zz(0, b, c)) for b in range(2)] for c in range(2)]
+    mutate(zz)
+    s = np.full((2, 2), 1e9)
+    g = np.full((2, 2), 2.0)
+    img = H.component_bytes(dims, 32, s, g, zz)
+    good = H.basis_from_ints(dims, 2, adversarial_zz)
+    paths = H.write_files([img, good[1], good[2]], str(tmp_path / "bad."))
+    return CompressedBasis.from_files(paths)
+
+
+@pytest.mark.parametrize("name,mutate,bit", [
+    ("run_length_too_long", lambda zz: zz[0].__setitem__(0, np.array([0, 0, 600], np.int32)), 0x1),
+    ("run_length_one", lambda zz: zz[0].__setitem__(0, np.concatenate([[4, 4, 1], np.arange(1, 512)]).astype(np.int32)), 0x1),
+    ("short_segment", lambda zz: zz[1].__setitem__(1, np.array([0, 0, 511], np.int32)), 0x2),
+    ("long_segment", lambda zz: zz[1].__setitem__(0, np.array([3, 0, 0, 512], np.int32)), 0x2),
+    ("truncated_run", lambda zz: zz[0].__setitem__(1, np.concatenate([np.arange(1, 511), [9, 9]]).astype(np.int32)), 0x4),
+])
+def test_malformed_streams_are_flagged(tmp_path, gpu, name, mutate, bit):
+    print(f"[case] malformed stream {name}", flush=True)
+    gb = _bad_basis(tmp_path, mutate)
+    gb.decode_block_ints(0, 0)
+    gb.decode_block_ints(0, 1)
+    bits = gb.check_stream_errors()
+    assert bits & bit, (name, bits)
+    v = DeviceArray.zeros((3 * gb.full_cells,), np.float32)
+    gb.project(v)
+    assert gb.check_stream_errors() & bit, name
