@@ -186,7 +186,7 @@ step_kernel(DevBasis B, const float* __restrict__ v_in, const double* __restrict
     if (UNPROJ) inverse_dct_block(field_out, B, k, b, C, zig, w.az, w.vz);
     wave_lds_sync();
   }
-  if (err) atomicOr(B.err, err);
+  if (err && lane == 0) atomicOr(B.err, err);
 
   if (PROJ) {
     __syncthreads();

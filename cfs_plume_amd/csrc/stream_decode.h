@@ -131,9 +131,13 @@ class StreamParser {
   }
 
   // Called after the last chunk: every segment must have produced 512 values.
+  // Returns the error bits of the whole wave (every lane gets the same value).
   __device__ uint32_t finish() {
     if (pos_ != nseg_ * BC || state_ != 0) err_ |= ERR_SEG_SIZE;
-    return err_;
+    uint32_t all = 0;
+    for (uint32_t bit = 1; bit <= ERR_SEG_INDEX; bit <<= 1)
+      if (__ballot((err_ & bit) != 0)) all |= bit;
+    return all;
   }
   __device__ uint32_t errors() const { return err_; }
 

@@ -285,29 +285,25 @@ struct Component {
   std::vector<double> dlist;             // (4 nBits + 1) x 512 damping list
 };
 
-int read_component(const char* path, Component* c) {
-  FILE* f = std::fopen(path, "rb");
-  if (!f) return ORC_ERR_IO;
+int parse_component(const uint8_t* d, size_t size, Component* c) {
+  if (size < 40) return ORC_ERR_FORMAT;
   int32_t h[10];
-  if (std::fread(h, sizeof(int32_t), 10, f) != 10) { std::fclose(f); return ORC_ERR_FORMAT; }
-  if (h[0] != 1) { std::fclose(f); return ORC_ERR_FORMAT; }  // v0 computes paddings wrongly :1506-1514
+  std::memcpy(h, d, 40);
+  if (h[0] != 1) return ORC_ERR_FORMAT;  // v0 computes paddings wrongly :1506-1514
   c->nBits = h[1]; c->X = h[2]; c->Y = h[3]; c->Z = h[4];
   c->Px = h[5]; c->Py = h[6]; c->Pz = h[7]; c->r = h[8]; c->nB = h[9];
-  if (c->r <= 0 || c->nB <= 0 || c->nBits <= 0 || c->nBits > 32) { std::fclose(f); return ORC_ERR_FORMAT; }
+  if (c->r <= 0 || c->nB <= 0 || c->nBits <= 0 || c->nBits > 32) return ORC_ERR_FORMAT;
   size_t n = (size_t)c->r * c->nB;
+  size_t meta = 40 + n * 24;
+  if (size < meta) return ORC_ERR_FORMAT;
   c->s.resize(n); c->gamma.resize(n); c->len.resize(n); c->idx.resize(n);
-  if (std::fread(c->s.data(), 8, n, f) != n || std::fread(c->gamma.data(), 8, n, f) != n ||
-      std::fread(c->len.data(), 4, n, f) != n || std::fread(c->idx.data(), 4, n, f) != n) {
-    std::fclose(f); return ORC_ERR_FORMAT;
-  }
-  long start = std::ftell(f);
-  std::fseek(f, 0, SEEK_END);
-  long end = std::ftell(f);
-  std::fseek(f, start, SEEK_SET);
-  size_t nt = (size_t)(end - start) / 4;
-  c->tokens.resize(nt + 4, 0);  // +4: the in-place decoders read 2 ints past a stream
-  if (std::fread(c->tokens.data(), 4, nt, f) != nt) { std::fclose(f); return ORC_ERR_FORMAT; }
-  std::fclose(f);
+  std::memcpy(c->s.data(), d + 40, n * 8);
+  std::memcpy(c->gamma.data(), d + 40 + n * 8, n * 8);
+  std::memcpy(c->len.data(), d + 40 + n * 16, n * 4);
+  std::memcpy(c->idx.data(), d + 40 + n * 20, n * 4);
+  size_t nt = (size - meta) / 4;
+  c->tokens.assign(nt + 4, 0);  // +4: the in-place decoders read 2 ints past a stream
+  std::memcpy(c->tokens.data(), d + meta, nt * 4);
   for (size_t i = 0; i < n; i++)
     if (c->len[i] < 0 || c->len[i] > 2 * BC || c->idx[i] < 0 || (size_t)c->idx[i] + c->len[i] > nt)
       return ORC_ERR_FORMAT;
@@ -318,6 +314,21 @@ int read_component(const char* path, Component* c) {
   for (int k = 0; k < nk; k++)
     for (int i = 0; i < BC; i++) c->dlist[(size_t)k * BC + i] = std::pow(T.damp[i], k / 4.0);
   return ORC_OK;
+}
+
+// ReadBinaryFileToMemory COMPRESSION.cpp:1456-1594
+int read_component(const char* path, Component* c) {
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return ORC_ERR_IO;
+  std::fseek(f, 0, SEEK_END);
+  long n = std::ftell(f);
+  std::fseek(f, 0, SEEK_SET);
+  if (n < 0) { std::fclose(f); return ORC_ERR_IO; }
+  std::vector<uint8_t> buf((size_t)n);
+  size_t got = n ? std::fread(buf.data(), 1, (size_t)n, f) : 0;
+  std::fclose(f);
+  if (got != (size_t)n) return ORC_ERR_IO;
+  return parse_component(buf.data(), buf.size(), c);
 }
 
 int write_component(const char* path, const Component& c) {
@@ -543,6 +554,23 @@ void* orc_basis_open(const char* p0, const char* p1, const char* p2, int* status
   return B;
 }
 void orc_basis_close(void* h) { delete (Basis*)h; }
+
+void* orc_basis_open_mem(const uint8_t* b0, size_t n0, const uint8_t* b1, size_t n1, const uint8_t* b2, size_t n2,
+                         int* status) {
+  Basis* B = new Basis;
+  const uint8_t* b[3] = {b0, b1, b2};
+  size_t n[3] = {n0, n1, n2};
+  for (int k = 0; k < 3; k++) {
+    int st = parse_component(b[k], n[k], &B->comp[k]);
+    if (st) {
+      delete B;
+      *status = st;
+      return nullptr;
+    }
+  }
+  *status = ORC_OK;
+  return B;
+}
 
 // info: nBits, X, Y, Z, Px, Py, Pz, r, nB, tokens0, tokens1, tokens2
 void orc_basis_info(void* h, int64_t* info) {

@@ -51,6 +51,8 @@ def _load():
         "orc_compress_matrix": (c_int, [_f64p, c_int, c_int, c_int, c_int, c_int, c_double, c_int, c_char_p]),
         "orc_basis_open": (c_void_p, [c_char_p, c_char_p, c_char_p, ctypes.POINTER(c_int)]),
         "orc_basis_close": (None, [c_void_p]),
+        "orc_basis_open_mem": (c_void_p, [c_char_p, ctypes.c_size_t, c_char_p, ctypes.c_size_t, c_char_p,
+                                          ctypes.c_size_t, ctypes.POINTER(c_int)]),
         "orc_basis_info": (None, [c_void_p, _i64p]),
         "orc_decode_block_ints": (c_int, [c_void_p, c_int, c_int, c_int, _i32p, ctypes.POINTER(c_int)]),
         "orc_decode_block_coeffs": (c_int, [c_void_p, c_int, c_int, c_int, c_int, _f64p]),
@@ -226,9 +228,13 @@ def compress_matrix(U_colmajor, dims, n_bits, percent, max_iterations, prefix: s
 class Basis:
     """Three component files read as by ReadBinaryFileToMemory x3."""
 
-    def __init__(self, paths):
+    def __init__(self, paths=None, buffers=None):
         st = ctypes.c_int(0)
-        h = lib().orc_basis_open(*(p.encode() for p in paths), ctypes.byref(st))
+        if buffers is not None:
+            b = [bytes(x) for x in buffers]
+            h = lib().orc_basis_open_mem(b[0], len(b[0]), b[1], len(b[1]), b[2], len(b[2]), ctypes.byref(st))
+        else:
+            h = lib().orc_basis_open(*(p.encode() for p in paths), ctypes.byref(st))
         if not h:
             raise OracleError(f"ReadBinaryFileToMemory failed with status {st.value}")
         self._h = h
