@@ -77,6 +77,11 @@ def main():
     ap.add_argument("--cpu-sample-cols", type=int, default=24)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--threads", type=int, default=16)
+    ap.add_argument("--cache", default="", help="directory to keep/reuse the encoded component files")
+    ap.add_argument("--mode", default="step", choices=["step", "project", "unproject"],
+                    help="time the fused step (default) or one half of it")
+    ap.add_argument("--copy-calibration-mb", type=int, default=0,
+                    help="also run one device-to-device copy of this size (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU split is a design "
@@ -90,15 +95,27 @@ def main():
 
     grid, r, percent = WORKLOADS[args.workload]
     dims = tuple(g - 2 for g in grid)
+    cache = [os.path.join(args.cache, f"U.{args.workload}.component{k}") for k in range(3)] if args.cache else None
+    if cache and all(os.path.exists(p) for p in cache):
+        bufs = [open(p, "rb").read() for p in cache]
+        log(f"[bench] reusing encoded basis from {args.cache}")
+    else:
+        t0 = time.perf_counter()
+        U = synth.synthetic_basis(dims, r, seed=7)
+        log(f"[bench] synthetic basis {dims} r={r}: {time.perf_counter() - t0:.1f}s")
+        t0 = time.perf_counter()
+        enc = C.encode_matrix(U, dims, 32, percent, 50, threads=args.threads)
+        log(f"[bench] encoded: {time.perf_counter() - t0:.1f}s")
+        del U
+        bufs = enc.buffers()
+        del enc
+        if cache:
+            os.makedirs(args.cache, exist_ok=True)
+            for p, b in zip(cache, bufs):
+                with open(p, "wb") as f:
+                    f.write(b)
     t0 = time.perf_counter()
-    U = synth.synthetic_basis(dims, r, seed=7)
-    log(f"[bench] synthetic basis {dims} r={r}: {time.perf_counter() - t0:.1f}s")
-    t0 = time.perf_counter()
-    enc = C.encode_matrix(U, dims, 32, percent, 50, threads=args.threads)
-    log(f"[bench] encoded: {time.perf_counter() - t0:.1f}s")
-    del U
-    t0 = time.perf_counter()
-    basis = enc.open()
+    basis = C.CompressedBasis.from_buffers(bufs)
     tokens = basis.tokens
     log(f"[bench] opened on device: {time.perf_counter() - t0:.1f}s tokens={tokens} "
         f"mean T/segment={sum(tokens) / (3 * r * basis.nB):.1f} device={basis.device_bytes / 1e9:.2f} GB")
@@ -111,8 +128,16 @@ def main():
     fo = DeviceArray.zeros((len(v),), np.float32)
     stream = Stream()
 
+    def run_once():
+        if args.mode == "step":
+            C.step(basis, basis, dv, qo, dq, fo, stream)
+        elif args.mode == "project":
+            basis.project(dv, qo, stream)
+        else:
+            basis.unproject(dq, fo, stream)
+
     for _ in range(args.warmup):
-        C.step(basis, basis, dv, qo, dq, fo, stream)
+        run_once()
     stream.synchronize()
     basis.raise_on_stream_errors()
 
@@ -122,7 +147,7 @@ def main():
     t_wall = time.perf_counter()
     ev[0].record(stream)
     for i in range(args.steps):
-        C.step(basis, basis, dv, qo, dq, fo, stream)
+        run_once()
         ev[i + 1].record(stream)
     stream.synchronize()
     synchronize()
@@ -134,7 +159,7 @@ def main():
     step_bytes = basis.step_bytes()
     kernel_ms = float(per.mean())
     achieved = step_bytes / (kernel_ms * 1e-3) / 1e9
-    log(f"[bench] {args.steps} steps: {ms_step:.4f} ms/step (events), wall {t_wall / args.steps * 1e3:.4f} ms/step, "
+    log(f"[bench] mode={args.mode} {args.steps} steps: {ms_step:.4f} ms/step (events), wall {t_wall / args.steps * 1e3:.4f} ms/step, "
         f"median launch {np.median(per):.4f} ms, B_step={step_bytes / 1e9:.3f} GB -> {achieved:.0f} GB/s")
 
     q_gpu = qo.numpy()
@@ -172,7 +197,7 @@ def main():
 
     if not args.no_cpu:
         ns = min(args.cpu_sample_cols, r)
-        sps, t_full, t_sample, q_s, f_s = cpu_baseline(enc.buffers(), v, q, r, ns)
+        sps, t_full, t_sample, q_s, f_s = cpu_baseline(bufs, v, q, r, ns)
         result["cpu_baseline"] = {
             "value": round(sps, 5), "unit": "steps/s", "cores": 1, "kind": "port",
             "sample": f"FP64 oracle project+unproject of columns 0..{ns - 1} of {r} "
@@ -191,6 +216,14 @@ def main():
             "field_rel_l2": float(np.linalg.norm(f_gpu_s - f_s) / np.linalg.norm(f_s)),
             "columns": ns,
         }
+    if args.copy_calibration_mb:
+        n = args.copy_calibration_mb << 20
+        src = DeviceArray.zeros((n,), np.uint8)
+        dst = DeviceArray((n,), np.uint8)
+        from cfs_plume_amd.hip import check, hip
+        check(hip().hipMemcpy(dst.ptr, src.ptr, n, 3), "hipMemcpy D2D")
+        synchronize()
+        log(f"[bench] copy calibration: {n} bytes device-to-device")
     print(json.dumps(result), flush=True)
 
 

@@ -54,7 +54,7 @@ class BasisInfo(ctypes.Structure):
         ("padded", ctypes.c_int32 * 3),
         ("num_cols", ctypes.c_int32),
         ("num_blocks", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("hazards", ctypes.c_int32),
         ("tokens", ctypes.c_int64 * 3),
         ("device_bytes", ctypes.c_int64),
     ]

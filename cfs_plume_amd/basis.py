@@ -27,10 +27,7 @@ from . import _lib
 from .hip import DeviceArray, Stream
 
 STREAM_BITS = {
-    0x1: "run length outside [2, 512]",
-    0x2: "segment does not decode to 512 values",
-    0x4: "segment ends inside a run",
-    0x8: "decoded positions past the last segment",
+    0x2: "a stream did not decode to 512 values per segment (device memory corrupted?)",
 }
 
 
@@ -108,6 +105,7 @@ class CompressedBasis:
         self.nB = info.num_blocks
         self.tokens = tuple(info.tokens)
         self.device_bytes = info.device_bytes
+        self.hazards = info.hazards
         X, Y, Z = self.dims
         self.full_cells = (X + 2) * (Y + 2) * (Z + 2)
         self.peeled_cells = X * Y * Z

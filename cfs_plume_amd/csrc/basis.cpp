@@ -115,6 +115,43 @@ void parallel_for(size_t n, const std::function<void(size_t, size_t)>& fn) {
   for (auto& x : th) x.join();
 }
 
+// Canonical RunLengthDecodeBinary parse (COMPRESSION.cpp:1198-1283) of one
+// segment, checking what the reference only asserts (:1206-1207, :1267) or
+// silently tolerates: run lengths in [2, 512], no run cut off by the end of
+// the segment, exactly 512 values.  Returns nullptr if the segment is valid.
+// Also returns vmax = max |int| * (1+u+v+w)^gamma over the segment, the bound
+// of its dequantised values times s (damp_row: the 22 powers for its gamma).
+const char* check_segment(const int32_t* t, int len, const double* damp_row, const uint8_t* dz,
+                          bool* ends_in_singleton, double* vmax) {
+  int i = 0, j = 0;
+  bool single = false;
+  double m = 0.0;
+  while (i < len) {
+    if (j >= BC) return "decodes to more than 512 values";
+    const double a = std::fabs((double)t[i]);
+    if (i + 1 < len && t[i] == t[i + 1]) {
+      if (i + 2 >= len) return "ends inside a run";
+      const int L = t[i + 2];
+      if (L < 2 || L > BC) return "run length outside [2, 512]";
+      if (j + L > BC) return "decodes to more than 512 values";
+      if (a > 0)
+        for (int q = j; q < j + L; q++) m = std::max(m, a * damp_row[dz[q]]);
+      j += L;
+      i += 3;
+      single = false;
+    } else {
+      m = std::max(m, a * damp_row[dz[j]]);
+      j++;
+      i++;
+      single = true;
+    }
+  }
+  if (j != BC) return "decodes to fewer than 512 values";
+  *ends_in_singleton = single;
+  *vmax = m;
+  return nullptr;
+}
+
 template <class T>
 int dev_upload(T** dst, const T* src, size_t count, size_t* total) {
   const size_t bytes = std::max<size_t>(count * sizeof(T), 4);
@@ -163,6 +200,7 @@ struct cfs_basis {
   int device = 0;
   int nBits = 0, X = 0, Y = 0, Z = 0, Px = 0, Py = 0, Pz = 0, r = 0, nB = 0, nk = 0;
   int64_t ntok[3] = {0, 0, 0};
+  int64_t nhazards = 0;
   int grid[4] = {0, 0, 0, 0};  // persistent grid per mode (index = mode)
   size_t device_bytes = 0;
   DevBasis dev{};
@@ -191,8 +229,6 @@ int build_basis(const uint8_t* const bufs[3], const size_t sizes[3], cfs_basis**
       return fail(CFS_ERR_FORMAT, "component headers disagree");
 
   std::unique_ptr<cfs_basis> B(new cfs_basis);
-  hipError_t e = hipGetDevice(&B->device);
-  if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
   B->nBits = f[0].nBits;
   B->X = f[0].X; B->Y = f[0].Y; B->Z = f[0].Z;
   B->Px = f[0].Px; B->Py = f[0].Py; B->Pz = f[0].Pz;
@@ -205,7 +241,7 @@ int build_basis(const uint8_t* const bufs[3], const size_t sizes[3], cfs_basis**
   std::vector<uint64_t> blk_base((size_t)3 * nB);
   std::vector<uint32_t> blk_len((size_t)3 * nB);
   std::vector<uint32_t> seg_lk(nseg), seg_off(nseg);
-  std::vector<float> seg_sinv(nseg);
+  std::vector<float> seg_sinv(nseg), seg_vmax(nseg);
   uint64_t pos = 0;
   for (int k = 0; k < 3; k++) {
     B->ntok[k] = (int64_t)f[k].ntok;
@@ -247,6 +283,47 @@ int build_basis(const uint8_t* const bufs[3], const size_t sizes[3], cfs_basis**
       }
     }
   });
+  // Check every segment; record the "hazard" boundaries of each block stream:
+  // a segment ending in a singleton equal to the next segment's first token.
+  std::vector<std::vector<uint32_t>> hz_lists((size_t)3 * nB);
+  std::vector<std::string> bad((size_t)3 * nB);
+  std::vector<double> dpow((size_t)(4 * B->nBits + 1) * NDIAG);
+  for (int k = 0; k <= 4 * B->nBits; k++)
+    for (int d = 0; d < NDIAG; d++) dpow[(size_t)k * NDIAG + d] = std::pow((double)(1 + d), k / 4.0);
+  const uint8_t* dzt = host_tables().dz;
+  parallel_for((size_t)3 * nB, [&](size_t a, size_t bnd) {
+    for (size_t kb = a; kb < bnd && bad[kb].empty(); kb++) {
+      const int k = (int)(kb / nB), b = (int)(kb % nB);
+      const int32_t* stream = tokens.data() + blk_base[kb];
+      for (int c = 0; c < r; c++) {
+        const size_t si = kb * r + c;
+        const int L = (int)(seg_lk[si] & 0xffffu);
+        bool single = false;
+        double vmax = 0.0;
+        const char* why = check_segment(stream + seg_off[si], L, dpow.data() + (size_t)(seg_lk[si] >> 16) * NDIAG,
+                                        dzt, &single, &vmax);
+        if (why) {
+          bad[kb] = "component " + std::to_string(k) + " block " + std::to_string(b) + " column " +
+                    std::to_string(c) + ": segment " + why;
+          break;
+        }
+        if (single && c + 1 < r && stream[seg_off[si] + L - 1] == stream[seg_off[si] + L])
+          hz_lists[kb].push_back(seg_off[si] + (uint32_t)L - 1);
+        // in the scaled units of the device damping table, rounded up
+        seg_vmax[si] = (float)(std::ldexp(vmax, -FACTOR_SHIFT) * (1.0 + 1e-6));
+      }
+    }
+  });
+  for (const auto& m : bad)
+    if (!m.empty()) return fail(CFS_ERR_FORMAT, m);
+  std::vector<uint32_t> hz_start((size_t)3 * nB + 1, 0), hz;
+  for (size_t kb = 0; kb < (size_t)3 * nB; kb++) {
+    hz_start[kb] = (uint32_t)hz.size();
+    hz.insert(hz.end(), hz_lists[kb].begin(), hz_lists[kb].end());
+  }
+  hz_start[(size_t)3 * nB] = (uint32_t)hz.size();
+  B->nhazards = (int64_t)hz.size();
+
   // work order: blocks with the longest streams first
   std::vector<int32_t> order(nB);
   std::iota(order.begin(), order.end(), 0);
@@ -265,6 +342,8 @@ int build_basis(const uint8_t* const bufs[3], const size_t sizes[3], cfs_basis**
   fill_tables(tab.data(), T.dct, T.zig, T.rev, T.dz);
 
   // persistent grid sizes
+  hipError_t e = hipGetDevice(&B->device);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
   int cus = 0;
   e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, B->device);
   if (e != hipSuccess) return hip_fail(e, "hipDeviceGetAttribute");
@@ -283,6 +362,8 @@ int build_basis(const uint8_t* const bufs[3], const size_t sizes[3], cfs_basis**
   int32_t* d_tok = nullptr;
   uint64_t* d_bb = nullptr;
   uint32_t *d_bl = nullptr, *d_lk = nullptr, *d_off = nullptr, *d_err = nullptr, *d_cnt = nullptr;
+  uint32_t *d_hzs = nullptr, *d_hz = nullptr;
+  float* d_vmax = nullptr;
   float *d_sinv = nullptr, *d_damp = nullptr;
   int32_t* d_order = nullptr;
   double* d_part = nullptr;
@@ -300,9 +381,12 @@ int build_basis(const uint8_t* const bufs[3], const size_t sizes[3], cfs_basis**
   UP(d_lk, seg_lk.data(), seg_lk.size());
   UP(d_off, seg_off.data(), seg_off.size());
   UP(d_sinv, seg_sinv.data(), seg_sinv.size());
+  UP(d_vmax, seg_vmax.data(), seg_vmax.size());
   UP(d_damp, damp.data(), damp.size());
   UP(d_tab, tab.data(), tab.size());
   UP(d_order, order.data(), order.size());
+  UP(d_hzs, hz_start.data(), hz_start.size());
+  UP(d_hz, hz.data(), hz.size());
   const uint32_t zeros[4] = {0, 0, 0, 0};
   UP(d_err, zeros, 1);
   UP(d_cnt, zeros, 2);
@@ -314,8 +398,11 @@ int build_basis(const uint8_t* const bufs[3], const size_t sizes[3], cfs_basis**
   D.seg_lk = d_lk;
   D.seg_off = d_off;
   D.seg_sinv = d_sinv;
+  D.seg_vmax = d_vmax;
   D.damp = d_damp;
   D.order = d_order;
+  D.hz_start = d_hzs;
+  D.hz = d_hz;
   D.err = d_err;
   D.counter = d_cnt;
   D.partials = d_part;
@@ -400,7 +487,7 @@ int cfs_basis_get_info(const cfs_basis* B, cfs_basis_info* info) {
   info->padded[0] = B->Px; info->padded[1] = B->Py; info->padded[2] = B->Pz;
   info->num_cols = B->r;
   info->num_blocks = B->nB;
-  info->reserved = 0;
+  info->hazards = (int32_t)B->nhazards;
   for (int k = 0; k < 3; k++) info->tokens[k] = B->ntok[k];
   info->device_bytes = (int64_t)B->device_bytes;
   return CFS_OK;

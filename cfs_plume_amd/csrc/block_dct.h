@@ -85,19 +85,20 @@ __device__ void forward_dct_block(const float* __restrict__ v, const DevBasis& B
   wave_lds_sync();
 }
 
-// Inverse orthonormal 3D DCT of the zigzag-ordered accumulator `az` and
+// Inverse orthonormal 3D DCT of the zigzag-ordered fixed-point accumulator
+// `az` (value = az * unit) and
 // scatter of the interior cells of component k into the full-grid field
 // (UnitaryBlockDCTEigen(-1) + AssimilateBlocksEigen + subfield + setWithPeeled,
 // COMPRESSION.cpp:2880-2898, VECTOR3_FIELD_3D.cpp:3415-3430).  Leaves az zero.
 __device__ void inverse_dct_block(float* __restrict__ field, const DevBasis& B, int k, int b,
-                                  const float* __restrict__ C, const uint16_t* zig, float* az, float* scr) {
+                                  const float* __restrict__ C, const uint16_t* zig, int* az, float unit, float* scr) {
   const int lane = lane_id();
   // gather + pass along z: lane = (u, v)
   {
     const int u = lane & 7, vv = lane >> 3;
     float a[BS];
 #pragma unroll
-    for (int w = 0; w < BS; w++) a[w] = az[padj(zig[u + 8 * vv + 64 * w])];
+    for (int w = 0; w < BS; w++) a[w] = (float)az[padj(zig[u + 8 * vv + 64 * w])] * unit;
 #pragma unroll
     for (int z = 0; z < BS; z++) {
       float s = 0.0f;
@@ -107,7 +108,7 @@ __device__ void inverse_dct_block(float* __restrict__ field, const DevBasis& B, 
     }
   }
   wave_lds_sync();
-  for (int i = lane; i < APAD; i += WAVE) az[i] = 0.0f;
+  for (int i = lane; i < APAD; i += WAVE) az[i] = 0;
   // pass along y: lane = (u, z)
   {
     const int u = lane & 7, z = lane >> 3;

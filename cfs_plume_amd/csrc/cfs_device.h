@@ -26,16 +26,14 @@ constexpr int BC = 512;
 constexpr int NDIAG = 22;      // distinct values of u+v+w
 constexpr int TPL = 4;         // tokens per lane per chunk in the stream decoder
 constexpr int CHUNK = 64 * TPL;
-constexpr int GUARD_TOKENS = 2 * CHUNK;
+constexpr int GUARD_TOKENS = 3 * CHUNK;  // covers the two-chunk prefetch
 constexpr int MAX_SEG_TOKENS = 2 * BC;  // reference assert COMPRESSION.cpp:1206
 constexpr int FACTOR_SHIFT = 64;
 
-// error bits raised by the device decoders
+// error bit raised by the device decoders; the loader checks every segment
+// before upload, so it signals corrupted device memory, not a bad file
 enum : uint32_t {
-  ERR_RUN_LENGTH = 1u << 0,   // run length outside [2, 512]
-  ERR_SEG_SIZE = 1u << 1,     // a segment does not decode to exactly 512 values
-  ERR_TRUNCATED = 1u << 2,    // a segment ends inside a run
-  ERR_SEG_INDEX = 1u << 3,    // decoded positions run past the last segment
+  ERR_INCOMPLETE = 1u << 1,   // a stream did not decode to 512 values per segment
 };
 
 struct DevBasis {
@@ -45,7 +43,10 @@ struct DevBasis {
   const uint32_t* seg_lk;     // [3*nB*r] length | k4 << 16, block-major
   const uint32_t* seg_off;    // [3*nB*r] offset of segment inside its block stream
   const float* seg_sinv;      // [3*nB*r] 2^FACTOR_SHIFT / s
+  const float* seg_vmax;      // [3*nB*r] max |int| * damp^gamma * 2^-FACTOR_SHIFT of the segment
   const float* damp;          // [nk * 22] (1+d)^(k/4) * 2^-FACTOR_SHIFT in FP32
+  const uint32_t* hz_start;   // [3*nB+1] first hazard of block stream (k, b) in hz
+  const uint32_t* hz;         // hazard offsets (segment-final singleton equal to the next token)
   const int32_t* order;       // [nB] blocks in descending work order
   uint32_t* err;              // device error bits
   uint32_t* counter;          // persistent-kernel work counter (left at 0 after each step)

@@ -26,7 +26,6 @@ __global__ void __launch_bounds__(DEC_WAVES * WAVE)
 decode_kernel(DevBasis B, DecodeArgs a, const float* __restrict__ g_damp, const Tables* __restrict__ g_tab) {
   __shared__ __align__(16) float buf[DEC_WAVES][BC];
   __shared__ __align__(16) float scr[DEC_WAVES][BC];
-  __shared__ int ends[DEC_WAVES];
   const int wave = threadIdx.x / WAVE;
   const int lane = lane_id();
   const long item = (long)blockIdx.x * DEC_WAVES + wave;
@@ -48,10 +47,10 @@ decode_kernel(DevBasis B, DecodeArgs a, const float* __restrict__ g_damp, const 
   float* bf = buf[wave];
   int* bi = reinterpret_cast<int*>(bf);
   for (int i = lane; i < BC; i += WAVE) bi[i] = 0;
-  if (lane == 0) ends[wave] = lo + len - 1;
   wave_lds_sync();
 
-  StreamParser parser(B.tokens + aligned, lo, lo + len, &ends[wave], 1);
+  // one segment: nothing follows it inside [lo, lo+len), so no resets
+  StreamParser parser(B.tokens + aligned, lo, lo + len, nullptr, 0);
   for (int base = 0; base < lo + len; base += CHUNK) {
     Chunk ch;
     parser.parse(base, ch);
@@ -69,8 +68,7 @@ decode_kernel(DevBasis B, DecodeArgs a, const float* __restrict__ g_damp, const 
       }
     }
   }
-  const uint32_t err = parser.finish();
-  if (err && lane == 0) atomicOr(B.err, err);
+  if (!parser.complete(1) && lane == 0) atomicOr(B.err, (uint32_t)ERR_INCOMPLETE);
   wave_lds_sync();
 
   if (OUT == DEC_INTS || OUT == DEC_COEFFS) {

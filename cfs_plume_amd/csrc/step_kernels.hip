@@ -26,20 +26,19 @@ namespace cfs {
 enum : int { MODE_PROJECT = 1, MODE_UNPROJECT = 2, MODE_BOTH = 3 };
 
 constexpr int WAVES_PER_WG = 4;
+static_assert(TPL == 4, "head/tail segment selection assumes at most two segments per lane");
+
 struct WaveLds {
+  int4* prm;     // [r] {22*k4, bits(2^64/s), bits(fixed-point weight of q_c), 0}
+  double* qacc;  // [rp] this wave's FP64 project accumulator
+  double* ends;  // [rp] running project sum of the block stream at the end of each segment
   float* vz;     // [APAD] DCT(v) block in zigzag order (padded index)
-  float* az;     // [APAD] unproject accumulator, zigzag order (padded index)
-  int* ends;     // [r] inclusive end offset of each segment in the block stream
-  int* row;      // [r] damping-table row offset 22*k4
-  float* sinv;   // [r] 1/s
-  float* qs;     // [r] q_c/s (unproject weight)
-  float* q32;    // [r] this block's FP32 project partials
-  double* qacc;  // [r] this wave's FP64 project accumulator
+  int* az;       // [APAD] unproject accumulator, zigzag order (padded index), fixed point
 };
 
 __host__ __device__ inline size_t step_wave_lds_bytes(int r) {
   const size_t rp = (size_t)((r + 1) & ~1);
-  return 2 * APAD * 4 + rp * (4 * 5 + 8);
+  return ((size_t)r * 16 + rp * 16 + 2 * APAD * 4 + 15) / 16 * 16;
 }
 __host__ __device__ inline size_t step_shared_lds_bytes(int nk) {
   return ((size_t)nk * NDIAG * 4 + 15) / 16 * 16 + BC * 2 + BC;
@@ -51,43 +50,40 @@ __host__ inline size_t step_lds_bytes(int r, int nk) {
 __device__ __forceinline__ WaveLds carve_wave(char* base, int r) {
   const int rp = (r + 1) & ~1;
   WaveLds w;
-  w.qacc = reinterpret_cast<double*>(base);
-  char* p = base + (size_t)rp * 8;
+  w.prm = reinterpret_cast<int4*>(base);
+  char* p = base + (size_t)r * 16;
+  w.qacc = reinterpret_cast<double*>(p); p += (size_t)rp * 8;
+  w.ends = reinterpret_cast<double*>(p); p += (size_t)rp * 8;
   w.vz = reinterpret_cast<float*>(p); p += APAD * 4;
-  w.az = reinterpret_cast<float*>(p); p += APAD * 4;
-  w.ends = reinterpret_cast<int*>(p); p += (size_t)rp * 4;
-  w.row = reinterpret_cast<int*>(p); p += (size_t)rp * 4;
-  w.sinv = reinterpret_cast<float*>(p); p += (size_t)rp * 4;
-  w.qs = reinterpret_cast<float*>(p); p += (size_t)rp * 4;
-  w.q32 = reinterpret_cast<float*>(p);
+  w.az = reinterpret_cast<int*>(p);
   return w;
 }
 
-// Loads the per-segment parameters of block stream (k, b) into LDS.
-__device__ void load_block_params(const DevBasis& B, int k, int b, const double* __restrict__ q_in,
-                                  const WaveLds& w, bool unproject) {
+// Loads the per-segment parameters of block stream (k, b) into LDS and
+// returns 1/S, the unit of the fixed-point unproject accumulator.
+//
+// Unproject accumulates A[j] = sum_c q_c D_c[j] with integer LDS atomics
+// (gfx950's LDS float atomics are far slower).  Every term is bounded by
+// |q_c| * vmax_c / s_c (vmax_c from the loader), so with
+// S = 2^30 / sum_c |q_c| vmax_c / s_c every partial sum of rounded terms stays
+// below 2^31; the rounding error is at most r/2 units of 1/S per entry.
+__device__ float load_block_params(const DevBasis& B, int k, int b, const double* __restrict__ q_in,
+                                   const WaveLds& w, bool unproject) {
   const int lane = lane_id();
   const size_t i0 = ((size_t)k * B.nB + b) * B.r;
-  int carry = 0;
-  for (int c0 = 0; c0 < B.r; c0 += WAVE) {
-    const int c = c0 + lane;
-    uint32_t lk = 0;
-    float sv = 0.0f;
-    if (c < B.r) {
-      lk = B.seg_lk[i0 + c];
-      sv = B.seg_sinv[i0 + c];
-    }
-    const int len = (int)(lk & 0xffffu);
-    const int incl = wave_scan_add(len);
-    if (c < B.r) {
-      w.ends[c] = carry + incl - 1;
-      w.row[c] = (int)(lk >> 16) * NDIAG;
-      w.sinv[c] = sv;
-      w.qs[c] = unproject ? (float)(q_in[c] * (double)sv) : 0.0f;
-      w.q32[c] = 0.0f;
-    }
-    carry += readlane(incl, WAVE - 1);
+  float bound = 0.0f;
+  if (unproject)
+    for (int c = lane; c < B.r; c += WAVE)
+      bound += fabsf((float)(q_in[c] * (double)B.seg_sinv[i0 + c])) * B.seg_vmax[i0 + c];
+  bound = wave_sum(bound) * 1.0001f;
+  const float S = bound > 0.0f ? 1073741824.0f / bound : 0.0f;
+  for (int c = lane; c < B.r; c += WAVE) {
+    const uint32_t lk = B.seg_lk[i0 + c];
+    const float sv = B.seg_sinv[i0 + c];
+    const float qs = unproject ? (float)(q_in[c] * (double)sv) * S : 0.0f;
+    w.prm[c] = make_int4((int)(lk >> 16) * NDIAG, __float_as_int(sv), __float_as_int(qs), 0);
   }
+  return S > 0.0f ? 1.0f / S : 0.0f;
 }
 
 template <int MODE>
@@ -114,12 +110,13 @@ step_kernel(DevBasis B, const float* __restrict__ v_in, const double* __restrict
     dz[i] = g_tab->dz[i];
   }
   for (int c = lane; c < B.r; c += WAVE) w.qacc[c] = 0.0;
-  for (int i = lane; i < APAD; i += WAVE) w.az[i] = 0.0f;
+  for (int i = lane; i < APAD; i += WAVE) w.az[i] = 0;
   __syncthreads();
   const float* C = g_tab->dct;
 
-  uint32_t err = 0;
+  bool complete = true;
   const int total = 3 * B.nB;
+  const int last_seg = B.r - 1;
   for (;;) {
     int item = 0;
     if (lane == 0) item = (int)atomicAdd(B.counter, 1u);
@@ -129,64 +126,100 @@ step_kernel(DevBasis B, const float* __restrict__ v_in, const double* __restrict
     const int k = item % 3;
     const size_t kb = (size_t)k * B.nB + b;
 
-    load_block_params(B, k, b, q_in, w, UNPROJ);
-    if (PROJ) forward_dct_block(v_in, B, k, b, C, zig, w.az, w.vz);
+    const float inv_scale = load_block_params(B, k, b, q_in, w, UNPROJ);
+    if (PROJ) forward_dct_block(v_in, B, k, b, C, zig, reinterpret_cast<float*>(w.az), w.vz);
     if (PROJ && UNPROJ)
-      for (int i = lane; i < APAD; i += WAVE) w.az[i] = 0.0f;  // az served as DCT scratch
+      for (int i = lane; i < APAD; i += WAVE) w.az[i] = 0;  // az served as DCT scratch
     wave_lds_sync();
 
     const int len = (int)B.blk_len[kb];
-    StreamParser parser(B.tokens + B.blk_base[kb], 0, len, w.ends, B.r);
+    const uint32_t h0 = B.hz_start[kb];
+    StreamParser parser(B.tokens + B.blk_base[kb], 0, len, B.hz + h0, (int)(B.hz_start[kb + 1] - h0));
+    double run = 0.0;  // project sum of the chunks before this one (wave-uniform)
     for (int base = 0; base < len; base += CHUNK) {
       Chunk ch;
       parser.parse(base, ch);
-      // this lane's head and tail segments
-      const int last_seg = B.r - 1;
-      int sh = min(ch.P[0] >> 9, last_seg);
+      // head and tail segment of this lane (a lane spans at most two)
+      const int sh = min(ch.P[0] >> 9, last_seg);
       int st = sh;
 #pragma unroll
       for (int i = 1; i < TPL; i++)
         if (ch.em[i] > 0) st = min(ch.P[i] >> 9, last_seg);
-      const int row_h = w.row[sh], row_t = w.row[st];
-      float qs_h = 0.0f, qs_t = 0.0f;
-      if (UNPROJ) {
-        qs_h = w.qs[sh];
-        qs_t = w.qs[st];
-      }
-      float dot_h = 0.0f, dot_t = 0.0f;
+      const int4 ph = w.prm[sh];
+      const int4 pt = w.prm[st];
+      // batched table lookups for the first value of every token
+      int jj[TPL], dd[TPL];
+      bool nz[TPL], hd[TPL];
 #pragma unroll
       for (int i = 0; i < TPL; i++) {
-        if (ch.em[i] > 0 && ch.v[i] != 0) {
-          const bool hd = (ch.P[i] >> 9) <= sh;
-          const int row = hd ? row_h : row_t;
-          const float qs = hd ? qs_h : qs_t;
-          const float vf0 = (float)ch.v[i];
-          for (int m = 0; m < ch.em[i]; m++) {
-            const int j = (ch.P[i] + m) & (BC - 1);
-            const int jj = padj(j);
-            const float vf = vf0 * damp[row + dz[j]];
-            if (PROJ) {
-              const float p = vf * w.vz[jj];
-              if (hd) dot_h += p; else dot_t += p;
+        nz[i] = ch.em[i] > 0 && ch.v[i] != 0;
+        hd[i] = (ch.P[i] >> 9) <= sh;
+        const int j = ch.P[i] & (BC - 1);
+        jj[i] = padj(j);
+        dd[i] = dz[j];
+      }
+      float vfs[TPL], vzs[TPL];
+#pragma unroll
+      for (int i = 0; i < TPL; i++) {
+        vfs[i] = damp[(hd[i] ? ph.x : pt.x) + dd[i]];
+        vzs[i] = PROJ ? w.vz[jj[i]] : 0.0f;
+      }
+      float prod[TPL];
+      bool more = false;
+#pragma unroll
+      for (int i = 0; i < TPL; i++) {
+        const float vf = nz[i] ? (float)ch.v[i] * vfs[i] : 0.0f;
+        // project: value * DCT(v) * (1/s)
+        prod[i] = vf * vzs[i] * __int_as_float(hd[i] ? ph.y : pt.y);
+        if (UNPROJ && nz[i]) atomicAdd(&w.az[jj[i]], __float2int_rn(vf * __int_as_float(hd[i] ? ph.z : pt.z)));
+        more |= nz[i] && ch.em[i] > 1;
+      }
+      // the remaining values of nonzero runs (rare: runs are mostly zeros)
+      if (__ballot(more)) {
+#pragma unroll
+        for (int i = 0; i < TPL; i++) {
+          if (nz[i] && ch.em[i] > 1) {
+            const int row = hd[i] ? ph.x : pt.x;
+            const float sv = __int_as_float(hd[i] ? ph.y : pt.y);
+            const float qs = __int_as_float(hd[i] ? ph.z : pt.z);
+            const float v0 = (float)ch.v[i];
+            for (int m = 1; m < ch.em[i]; m++) {
+              const int j = (ch.P[i] + m) & (BC - 1);
+              const int jp = padj(j);
+              const float vf = v0 * damp[row + dz[j]];
+              if (PROJ) prod[i] += vf * w.vz[jp] * sv;
+              if (UNPROJ) atomicAdd(&w.az[jp], __float2int_rn(vf * qs));
             }
-            if (UNPROJ) atomicAdd(&w.az[jj], vf * qs);
           }
         }
       }
       if (PROJ) {
-        if (dot_h != 0.0f) atomicAdd(&w.q32[sh], dot_h * w.sinv[sh]);
-        if (st != sh && dot_t != 0.0f) atomicAdd(&w.q32[st], dot_t * w.sinv[st]);
+        // Running sum over the stream; the token that closes a segment (its
+        // emission ends on a multiple of 512) records the sum at that point.
+        float pre[TPL], acc = 0.0f;
+#pragma unroll
+        for (int i = 0; i < TPL; i++) {
+          acc += prod[i];
+          pre[i] = acc;
+        }
+        const float incl = wave_scan_addf(acc);
+        const float excl = incl - acc;
+#pragma unroll
+        for (int i = 0; i < TPL; i++)
+          if (ch.em[i] > 0 && ((ch.P[i] + ch.em[i]) & (BC - 1)) == 0)
+            w.ends[min(ch.P[i] >> 9, last_seg)] = run + (double)(excl + pre[i]);
+        run += (double)readlanef(incl, WAVE - 1);
       }
     }
-    err |= parser.finish();
+    complete = complete && parser.complete(B.r);
     wave_lds_sync();
 
     if (PROJ)
-      for (int c = lane; c < B.r; c += WAVE) w.qacc[c] += (double)w.q32[c];
-    if (UNPROJ) inverse_dct_block(field_out, B, k, b, C, zig, w.az, w.vz);
+      for (int c = lane; c < B.r; c += WAVE) w.qacc[c] += w.ends[c] - (c > 0 ? w.ends[c - 1] : 0.0);
+    if (UNPROJ) inverse_dct_block(field_out, B, k, b, C, zig, w.az, inv_scale, w.vz);
     wave_lds_sync();
   }
-  if (err && lane == 0) atomicOr(B.err, err);
+  if (!complete && lane == 0) atomicOr(B.err, (uint32_t)ERR_INCOMPLETE);
 
   if (PROJ) {
     __syncthreads();

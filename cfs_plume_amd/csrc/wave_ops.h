@@ -70,6 +70,17 @@ __device__ __forceinline__ int wave_scan_add(int x) {
   return x;
 }
 
+// Inclusive float prefix sum across the wave.
+__device__ __forceinline__ float wave_scan_addf(float x) {
+  x += dppf<DPP_ROW_SHR1>(0.0f, x);
+  x += dppf<DPP_ROW_SHR2>(0.0f, x);
+  x += dppf<DPP_ROW_SHR4>(0.0f, x);
+  x += dppf<DPP_ROW_SHR8>(0.0f, x);
+  x += dppf<DPP_ROW_BCAST15, 0xa>(0.0f, x);
+  x += dppf<DPP_ROW_BCAST31, 0xc>(0.0f, x);
+  return x;
+}
+
 // Value held by lane+1 (lane 63 receives `fill`).
 __device__ __forceinline__ uint32_t wave_shl1(uint32_t x, uint32_t fill) { return dpp<DPP_WAVE_SHL1>(fill, x); }
 // Value held by lane-1 (lane 0 receives `fill`).

@@ -25,14 +25,14 @@ extern "C" {
 #define CFS_ERR_IO (-2)      /* file could not be opened / read / written */
 #define CFS_ERR_FORMAT (-3)  /* malformed component file */
 #define CFS_ERR_HIP (-4)     /* HIP runtime error */
-#define CFS_ERR_STREAM (-5)  /* the device decoder found a malformed token stream */
+#define CFS_ERR_STREAM (-5)  /* the device decoder found an incomplete token stream */
 #define CFS_ERR_NOMEM (-6)   /* host or device allocation failed */
 
-/* device stream-error bits reported by cfs_check_stream_errors */
-#define CFS_STREAM_RUN_LENGTH 0x1u /* run length outside [2, 512] */
-#define CFS_STREAM_SEG_SIZE 0x2u   /* a segment does not decode to 512 values */
-#define CFS_STREAM_TRUNCATED 0x4u  /* a segment ends inside a run */
-#define CFS_STREAM_SEG_INDEX 0x8u  /* decoded positions past the last segment */
+/* device stream-error bit reported by cfs_check_stream_errors.  The loaders
+ * parse every segment before upload and reject files whose segments do not
+ * decode to exactly 512 values, have run lengths outside [2, 512] or end
+ * inside a run (CFS_ERR_FORMAT), so this bit signals corrupted device data. */
+#define CFS_STREAM_INCOMPLETE 0x2u
 
 typedef struct cfs_basis cfs_basis;
 typedef struct cfs_encoded cfs_encoded;
@@ -43,7 +43,7 @@ typedef struct {
   int32_t padded[3];     /* padded to multiples of 8 */
   int32_t num_cols;      /* r */
   int32_t num_blocks;    /* nB per component */
-  int32_t reserved;
+  int32_t hazards;       /* boundaries where a singleton ends a segment and equals the next token */
   int64_t tokens[3];     /* int32 tokens per component file */
   int64_t device_bytes;  /* device memory held by the handle */
 } cfs_basis_info;
@@ -59,7 +59,9 @@ int cfs_abi_version(void);
  *  SUBSPACE_FLUID_3D_COMPRESSED_EIGEN.cpp:1488-1520, 1385-1393).
  * Reads the three component files (format v1), validates them, regroups the
  * tokens block-major and uploads them to the current HIP device.  Where the
- * reference returns NULL and prints, these return CFS_ERR_IO/FORMAT. */
+ * reference returns NULL and prints, these return CFS_ERR_IO/FORMAT; where it
+ * asserts on a token stream (COMPRESSION.cpp:1206-1207, 1267) or decodes a
+ * short segment with trailing zeros, they return CFS_ERR_FORMAT. */
 int cfs_basis_open_files(const char* path_x, const char* path_y, const char* path_z, cfs_basis** out);
 /* Same from three in-memory file images (e.g. cfs_encoded_buffer). */
 int cfs_basis_open_memory(const void* buf_x, size_t size_x, const void* buf_y, size_t size_y, const void* buf_z,
@@ -107,7 +109,7 @@ int cfs_decode_block_coeffs(cfs_basis* basis, int comp, int col, int block0, int
 int cfs_get_submatrix(cfs_basis* basis, int start_row, double* out, void* stream);
 
 /* Synchronises, returns the CFS_STREAM_* bits raised since the last call and
- * clears them.  Replaces the reference's asserts (COMPRESSION.cpp:1206,1267). */
+ * clears them. */
 int cfs_check_stream_errors(cfs_basis* basis, uint32_t* bits);
 
 /* ---- encoder (host) ---------------------------------------------------------

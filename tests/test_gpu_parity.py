@@ -84,7 +84,31 @@ def adversarial(tmp_path_factory, gpu):
     return _pair(bufs, tmp_path_factory, "adv")
 
 
-CASES = ["smooth", "rough", "adversarial"]
+def hazard_zz(k, b, c):
+    """Every segment starts and ends with the singleton 5 (z[510] != 5), so each
+    boundary inside a block stream reads [.., 5][5, ..]: without a reset the
+    parser would take the pair for a run."""
+    z = (np.arange(512, dtype=np.int32) % 97) + 10 + 3 * k
+    z[0] = 5
+    z[511] = 5
+    if c % 2:
+        z[200:300] = 0  # a zero run in the middle of every other column
+    return z
+
+
+@pytest.fixture(scope="module")
+def hazard(tmp_path_factory, gpu):
+    dims = (16, 8, 8)
+    print(f"[case] hazard boundaries dims={dims} r=6", flush=True)
+    return _pair(H.basis_from_ints(dims, 6, hazard_zz, seed=12), tmp_path_factory, "hazard")
+
+
+CASES = ["smooth", "rough", "adversarial", "hazard"]
+
+
+def test_hazards_are_listed(hazard):
+    ob, gb = hazard
+    assert gb.hazards == 3 * gb.nB * (gb.r - 1)
 
 
 @pytest.mark.parametrize("case", CASES)
@@ -195,32 +219,3 @@ def test_submatrix(case, request):
         assert rel(got, want) <= TOL, (cell, rel(got, want))
 
 
-def _bad_basis(tmp_path, mutate):
-    dims = (9, 8, 8)  # 2 blocks
-    zz = [[O.rle_encode(adversarial_zz(0, b, c)) for b in range(2)] for c in range(2)]
-    mutate(zz)
-    s = np.full((2, 2), 1e9)
-    g = np.full((2, 2), 2.0)
-    img = H.component_bytes(dims, 32, s, g, zz)
-    good = H.basis_from_ints(dims, 2, adversarial_zz)
-    paths = H.write_files([img, good[1], good[2]], str(tmp_path / "bad."))
-    return CompressedBasis.from_files(paths)
-
-
-@pytest.mark.parametrize("name,mutate,bit", [
-    ("run_length_too_long", lambda zz: zz[0].__setitem__(0, np.array([0, 0, 600], np.int32)), 0x1),
-    ("run_length_one", lambda zz: zz[0].__setitem__(0, np.concatenate([[4, 4, 1], np.arange(1, 512)]).astype(np.int32)), 0x1),
-    ("short_segment", lambda zz: zz[1].__setitem__(1, np.array([0, 0, 511], np.int32)), 0x2),
-    ("long_segment", lambda zz: zz[1].__setitem__(0, np.array([3, 0, 0, 512], np.int32)), 0x2),
-    ("truncated_run", lambda zz: zz[0].__setitem__(1, np.concatenate([np.arange(1, 511), [9, 9]]).astype(np.int32)), 0x4),
-])
-def test_malformed_streams_are_flagged(tmp_path, gpu, name, mutate, bit):
-    print(f"[case] malformed stream {name}", flush=True)
-    gb = _bad_basis(tmp_path, mutate)
-    gb.decode_block_ints(0, 0)
-    gb.decode_block_ints(0, 1)
-    bits = gb.check_stream_errors()
-    assert bits & bit, (name, bits)
-    v = DeviceArray.zeros((3 * gb.full_cells,), np.float32)
-    gb.project(v)
-    assert gb.check_stream_errors() & bit, name

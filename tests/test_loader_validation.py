@@ -1,0 +1,60 @@
+"""Loader validation on the host (no GPU needed): malformed token streams are
+rejected with CFS_ERR_FORMAT before anything is uploaded."""
+import numpy as np
+import pytest
+
+from cfs_plume_amd import CompressedBasis
+from oracle import oracle as O
+from tests import helpers as H
+from tests.test_gpu_parity import adversarial_zz
+
+
+def _bad_basis(tmp_path, mutate):
+    dims = (9, 8, 8)  # 2 blocks
+    zz = [[O.rle_encode(adversarial_zz(0, b, c)) for b in range(2)] for c in range(2)]
+    mutate(zz)
+    s = np.full((2, 2), 1e9)
+    g = np.full((2, 2), 2.0)
+    img = H.component_bytes(dims, 32, s, g, zz)
+    good = H.basis_from_ints(dims, 2, adversarial_zz)
+    return H.write_files([img, good[1], good[2]], str(tmp_path / "bad."))
+
+
+@pytest.mark.parametrize("name,mutate,reason", [
+    ("run_length_too_long", lambda zz: zz[0].__setitem__(0, np.array([0, 0, 600], np.int32)), "run length"),
+    ("run_length_one", lambda zz: zz[0].__setitem__(0, np.concatenate([[4, 4, 1], np.arange(1, 512)]).astype(np.int32)),
+     "run length"),
+    ("short_segment", lambda zz: zz[1].__setitem__(1, np.array([0, 0, 511], np.int32)), "fewer than 512"),
+    ("long_segment", lambda zz: zz[1].__setitem__(0, np.array([3, 0, 0, 512], np.int32)), "more than 512"),
+    ("truncated_run", lambda zz: zz[0].__setitem__(1, np.concatenate([np.arange(1, 511), [9, 9]]).astype(np.int32)),
+     "inside a run"),
+])
+def test_malformed_streams_are_rejected(tmp_path, name, mutate, reason):
+    """Where the reference asserts (COMPRESSION.cpp:1206-1207, 1267) or fills a
+    short segment with zeros, the loader refuses the file."""
+    from cfs_plume_amd import CfsError
+    paths = _bad_basis(tmp_path, mutate)
+    with pytest.raises(CfsError) as e:
+        CompressedBasis.from_files(paths)
+    assert "CFS_ERR_FORMAT" in str(e.value) and reason in str(e.value), str(e.value)
+    # the oracle decoder agrees that the segment is not a valid 512-value block
+    bad = 0
+    for b in range(2):
+        for c in range(2):
+            toks = ob_segment(paths[0], b, c)
+            if O.rle_decode_status(toks) != 0 or O.rle_decode(toks)[1]:
+                bad += 1
+    assert bad == 1
+
+
+def ob_segment(path, b, c):
+    import struct
+    raw = open(path, "rb").read()
+    h = struct.unpack("<10i", raw[:40])
+    r, nB = h[8], h[9]
+    m = r * nB
+    lens = np.frombuffer(raw, np.int32, m, 40 + 16 * m)
+    idx = np.frombuffer(raw, np.int32, m, 40 + 20 * m)
+    tok = np.frombuffer(raw, np.int32, offset=40 + 24 * m)
+    e = b + nB * c
+    return tok[idx[e]: idx[e] + lens[e]]
