@@ -29,6 +29,7 @@ namespace cfs {
 hipError_t launch_step(const DevBasis& B, int mode, const float* v_in, const double* q_in, double* q_out,
                        float* field_out, const float* d_damp, const void* d_tab, int grid, hipStream_t s);
 hipError_t step_occupancy(int mode, int r, int nk, int* wg_per_cu);
+int step_max_cols();
 size_t tables_bytes();
 void fill_tables(void* host_tab, const float* dct, const uint16_t* zig, const uint16_t* rev, const uint8_t* dz);
 hipError_t launch_decode(const DevBasis& B, int mode, const DecodeArgs& a, const float* d_damp, const void* d_tab,
@@ -223,6 +224,9 @@ int build_basis(const uint8_t* const bufs[3], const size_t sizes[3], cfs_basis**
     int st = parse_file(bufs[k], sizes[k], k, &f[k]);
     if (st) return st;
   }
+  if (f[0].r > step_max_cols())
+    return fail(CFS_ERR_FORMAT, "numCols " + std::to_string(f[0].r) + " exceeds the supported maximum " +
+                                    std::to_string(step_max_cols()));
   for (int k = 1; k < 3; k++)
     if (f[k].nBits != f[0].nBits || f[k].X != f[0].X || f[k].Y != f[0].Y || f[k].Z != f[0].Z || f[k].r != f[0].r ||
         f[k].nB != f[0].nB)

@@ -24,9 +24,9 @@ namespace cfs {
 constexpr int BS = 8;
 constexpr int BC = 512;
 constexpr int NDIAG = 22;      // distinct values of u+v+w
-constexpr int TPL = 4;         // tokens per lane per chunk in the stream decoder
-constexpr int CHUNK = 64 * TPL;
-constexpr int GUARD_TOKENS = 3 * CHUNK;  // covers the two-chunk prefetch
+constexpr int STEP_TPL = 8;    // tokens per lane per chunk in the fused step
+constexpr int DEC_TPL = 4;     // tokens per lane per chunk in the random-access decoders
+constexpr int GUARD_TOKENS = 3 * 64 * STEP_TPL;  // covers the two-chunk prefetch
 constexpr int MAX_SEG_TOKENS = 2 * BC;  // reference assert COMPRESSION.cpp:1206
 constexpr int FACTOR_SHIFT = 64;
 

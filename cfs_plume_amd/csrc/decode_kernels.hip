@@ -50,12 +50,12 @@ decode_kernel(DevBasis B, DecodeArgs a, const float* __restrict__ g_damp, const 
   wave_lds_sync();
 
   // one segment: nothing follows it inside [lo, lo+len), so no resets
-  StreamParser parser(B.tokens + aligned, lo, lo + len, nullptr, 0);
-  for (int base = 0; base < lo + len; base += CHUNK) {
-    Chunk ch;
+  StreamParser<DEC_TPL> parser(B.tokens + aligned, lo, lo + len, nullptr, 0);
+  for (int base = 0; base < lo + len; base += 64 * DEC_TPL) {
+    Chunk<DEC_TPL> ch;
     parser.parse(base, ch);
 #pragma unroll
-    for (int i = 0; i < TPL; i++) {
+    for (int i = 0; i < DEC_TPL; i++) {
       if (ch.em[i] > 0 && ch.v[i] != 0) {
         for (int m = 0; m < ch.em[i]; m++) {
           const int j = (ch.P[i] + m) & (BC - 1);

@@ -26,11 +26,13 @@ namespace cfs {
 enum : int { MODE_PROJECT = 1, MODE_UNPROJECT = 2, MODE_BOTH = 3 };
 
 constexpr int WAVES_PER_WG = 4;
-static_assert(TPL == 4, "head/tail segment selection assumes at most two segments per lane");
+constexpr int NT = STEP_TPL;
+constexpr int CH = 64 * NT;
+constexpr int QREG = 4;  // FP64 q accumulators per lane: columns lane + 64*m, r <= 256
+constexpr int MAX_COLS = 64 * QREG;
 
 struct WaveLds {
   int4* prm;     // [r] {22*k4, bits(2^64/s), bits(fixed-point weight of q_c), 0}
-  double* qacc;  // [rp] this wave's FP64 project accumulator
   double* ends;  // [rp] running project sum of the block stream at the end of each segment
   float* vz;     // [APAD] DCT(v) block in zigzag order (padded index)
   int* az;       // [APAD] unproject accumulator, zigzag order (padded index), fixed point
@@ -38,13 +40,15 @@ struct WaveLds {
 
 __host__ __device__ inline size_t step_wave_lds_bytes(int r) {
   const size_t rp = (size_t)((r + 1) & ~1);
-  return ((size_t)r * 16 + rp * 16 + 2 * APAD * 4 + 15) / 16 * 16;
+  return ((size_t)r * 16 + rp * 8 + 2 * APAD * 4 + 15) / 16 * 16;
 }
 __host__ __device__ inline size_t step_shared_lds_bytes(int nk) {
   return ((size_t)nk * NDIAG * 4 + 15) / 16 * 16 + BC * 2 + BC;
 }
 __host__ inline size_t step_lds_bytes(int r, int nk) {
-  return step_shared_lds_bytes(nk) + WAVES_PER_WG * step_wave_lds_bytes(r);
+  const size_t per_waves = WAVES_PER_WG * step_wave_lds_bytes(r);
+  const size_t red = WAVES_PER_WG * MAX_COLS * 8;  // end-of-kernel q reduction reuses the wave area
+  return step_shared_lds_bytes(nk) + (per_waves > red ? per_waves : red);
 }
 
 __device__ __forceinline__ WaveLds carve_wave(char* base, int r) {
@@ -52,7 +56,6 @@ __device__ __forceinline__ WaveLds carve_wave(char* base, int r) {
   WaveLds w;
   w.prm = reinterpret_cast<int4*>(base);
   char* p = base + (size_t)r * 16;
-  w.qacc = reinterpret_cast<double*>(p); p += (size_t)rp * 8;
   w.ends = reinterpret_cast<double*>(p); p += (size_t)rp * 8;
   w.vz = reinterpret_cast<float*>(p); p += APAD * 4;
   w.az = reinterpret_cast<int*>(p);
@@ -66,7 +69,7 @@ __device__ __forceinline__ WaveLds carve_wave(char* base, int r) {
 // (gfx950's LDS float atomics are far slower).  Every term is bounded by
 // |q_c| * vmax_c / s_c (vmax_c from the loader), so with
 // S = 2^30 / sum_c |q_c| vmax_c / s_c every partial sum of rounded terms stays
-// below 2^31; the rounding error is at most r/2 units of 1/S per entry.
+// below 2^31; the rounding error is below r units of 1/S per entry.
 __device__ float load_block_params(const DevBasis& B, int k, int b, const double* __restrict__ q_in,
                                    const WaveLds& w, bool unproject) {
   const int lane = lane_id();
@@ -101,19 +104,20 @@ step_kernel(DevBasis B, const float* __restrict__ v_in, const double* __restrict
   uint8_t* dz = reinterpret_cast<uint8_t*>(lds + dbytes + BC * 2);
   const int wave = threadIdx.x / WAVE;
   const int lane = lane_id();
-  char* wbase = lds + step_shared_lds_bytes(nk) + (size_t)wave * step_wave_lds_bytes(B.r);
-  const WaveLds w = carve_wave(wbase, B.r);
+  const WaveLds w = carve_wave(lds + step_shared_lds_bytes(nk) + (size_t)wave * step_wave_lds_bytes(B.r), B.r);
 
   for (int i = threadIdx.x; i < nk * NDIAG; i += blockDim.x) damp[i] = g_damp[i];
   for (int i = threadIdx.x; i < BC; i += blockDim.x) {
     zig[i] = g_tab->zig[i];
     dz[i] = g_tab->dz[i];
   }
-  for (int c = lane; c < B.r; c += WAVE) w.qacc[c] = 0.0;
   for (int i = lane; i < APAD; i += WAVE) w.az[i] = 0;
   __syncthreads();
   const float* C = g_tab->dct;
 
+  double qacc[QREG];
+#pragma unroll
+  for (int m = 0; m < QREG; m++) qacc[m] = 0.0;
   bool complete = true;
   const int total = 3 * B.nB;
   const int last_seg = B.r - 1;
@@ -134,61 +138,48 @@ step_kernel(DevBasis B, const float* __restrict__ v_in, const double* __restrict
 
     const int len = (int)B.blk_len[kb];
     const uint32_t h0 = B.hz_start[kb];
-    StreamParser parser(B.tokens + B.blk_base[kb], 0, len, B.hz + h0, (int)(B.hz_start[kb + 1] - h0));
+    StreamParser<NT> parser(B.tokens + B.blk_base[kb], 0, len, B.hz + h0, (int)(B.hz_start[kb + 1] - h0));
     double run = 0.0;  // project sum of the chunks before this one (wave-uniform)
-    for (int base = 0; base < len; base += CHUNK) {
-      Chunk ch;
+    for (int base = 0; base < len; base += CH) {
+      Chunk<NT> ch;
       parser.parse(base, ch);
-      // head and tail segment of this lane (a lane spans at most two)
-      const int sh = min(ch.P[0] >> 9, last_seg);
-      int st = sh;
+      // per-token segment parameters and table lookups for the first value
+      int4 pr[NT];
+      int jj[NT], dd[NT];
+      bool nz[NT];
 #pragma unroll
-      for (int i = 1; i < TPL; i++)
-        if (ch.em[i] > 0) st = min(ch.P[i] >> 9, last_seg);
-      const int4 ph = w.prm[sh];
-      const int4 pt = w.prm[st];
-      // batched table lookups for the first value of every token
-      int jj[TPL], dd[TPL];
-      bool nz[TPL], hd[TPL];
-#pragma unroll
-      for (int i = 0; i < TPL; i++) {
+      for (int i = 0; i < NT; i++) {
         nz[i] = ch.em[i] > 0 && ch.v[i] != 0;
-        hd[i] = (ch.P[i] >> 9) <= sh;
+        pr[i] = w.prm[min(ch.P[i] >> 9, last_seg)];
         const int j = ch.P[i] & (BC - 1);
         jj[i] = padj(j);
         dd[i] = dz[j];
       }
-      float vfs[TPL], vzs[TPL];
-#pragma unroll
-      for (int i = 0; i < TPL; i++) {
-        vfs[i] = damp[(hd[i] ? ph.x : pt.x) + dd[i]];
-        vzs[i] = PROJ ? w.vz[jj[i]] : 0.0f;
-      }
-      float prod[TPL];
+      float prod[NT];
       bool more = false;
 #pragma unroll
-      for (int i = 0; i < TPL; i++) {
-        const float vf = nz[i] ? (float)ch.v[i] * vfs[i] : 0.0f;
-        // project: value * DCT(v) * (1/s)
-        prod[i] = vf * vzs[i] * __int_as_float(hd[i] ? ph.y : pt.y);
-        if (UNPROJ && nz[i]) atomicAdd(&w.az[jj[i]], __float2int_rn(vf * __int_as_float(hd[i] ? ph.z : pt.z)));
+      for (int i = 0; i < NT; i++) {
+        const float dm = damp[pr[i].x + dd[i]];
+        const float vzv = PROJ ? w.vz[jj[i]] : 0.0f;
+        const float vf = nz[i] ? (float)ch.v[i] * dm : 0.0f;
+        prod[i] = vf * vzv * __int_as_float(pr[i].y);  // value * DCT(v), value in units of 1/s
+        if (UNPROJ && nz[i]) atomicAdd(&w.az[jj[i]], (int)(vf * __int_as_float(pr[i].z)));
         more |= nz[i] && ch.em[i] > 1;
       }
       // the remaining values of nonzero runs (rare: runs are mostly zeros)
       if (__ballot(more)) {
 #pragma unroll
-        for (int i = 0; i < TPL; i++) {
+        for (int i = 0; i < NT; i++) {
           if (nz[i] && ch.em[i] > 1) {
-            const int row = hd[i] ? ph.x : pt.x;
-            const float sv = __int_as_float(hd[i] ? ph.y : pt.y);
-            const float qs = __int_as_float(hd[i] ? ph.z : pt.z);
+            const float sv = __int_as_float(pr[i].y);
+            const float qs = __int_as_float(pr[i].z);
             const float v0 = (float)ch.v[i];
             for (int m = 1; m < ch.em[i]; m++) {
               const int j = (ch.P[i] + m) & (BC - 1);
               const int jp = padj(j);
-              const float vf = v0 * damp[row + dz[j]];
+              const float vf = v0 * damp[pr[i].x + dz[j]];
               if (PROJ) prod[i] += vf * w.vz[jp] * sv;
-              if (UNPROJ) atomicAdd(&w.az[jp], __float2int_rn(vf * qs));
+              if (UNPROJ) atomicAdd(&w.az[jp], (int)(vf * qs));
             }
           }
         }
@@ -196,16 +187,16 @@ step_kernel(DevBasis B, const float* __restrict__ v_in, const double* __restrict
       if (PROJ) {
         // Running sum over the stream; the token that closes a segment (its
         // emission ends on a multiple of 512) records the sum at that point.
-        float pre[TPL], acc = 0.0f;
+        float pre[NT], acc = 0.0f;
 #pragma unroll
-        for (int i = 0; i < TPL; i++) {
+        for (int i = 0; i < NT; i++) {
           acc += prod[i];
           pre[i] = acc;
         }
         const float incl = wave_scan_addf(acc);
         const float excl = incl - acc;
 #pragma unroll
-        for (int i = 0; i < TPL; i++)
+        for (int i = 0; i < NT; i++)
           if (ch.em[i] > 0 && ((ch.P[i] + ch.em[i]) & (BC - 1)) == 0)
             w.ends[min(ch.P[i] >> 9, last_seg)] = run + (double)(excl + pre[i]);
         run += (double)readlanef(incl, WAVE - 1);
@@ -214,22 +205,28 @@ step_kernel(DevBasis B, const float* __restrict__ v_in, const double* __restrict
     complete = complete && parser.complete(B.r);
     wave_lds_sync();
 
-    if (PROJ)
-      for (int c = lane; c < B.r; c += WAVE) w.qacc[c] += w.ends[c] - (c > 0 ? w.ends[c - 1] : 0.0);
+    if (PROJ) {
+#pragma unroll
+      for (int m = 0; m < QREG; m++) {
+        const int c = lane + WAVE * m;
+        if (c < B.r) qacc[m] += w.ends[c] - (c > 0 ? w.ends[c - 1] : 0.0);
+      }
+    }
     if (UNPROJ) inverse_dct_block(field_out, B, k, b, C, zig, w.az, inv_scale, w.vz);
     wave_lds_sync();
   }
   if (!complete && lane == 0) atomicOr(B.err, (uint32_t)ERR_INCOMPLETE);
 
   if (PROJ) {
+    // combine the waves' FP64 accumulators in a fixed order (through LDS)
     __syncthreads();
-    // combine the waves' FP64 accumulators in a fixed order
+    double* red = reinterpret_cast<double*>(lds + step_shared_lds_bytes(nk));  // [WAVES_PER_WG][MAX_COLS]
+#pragma unroll
+    for (int m = 0; m < QREG; m++) red[wave * MAX_COLS + lane + WAVE * m] = qacc[m];
+    __syncthreads();
     for (int c = threadIdx.x; c < B.r; c += blockDim.x) {
       double s = 0.0;
-      for (int wv = 0; wv < WAVES_PER_WG; wv++) {
-        const WaveLds o = carve_wave(lds + step_shared_lds_bytes(nk) + (size_t)wv * step_wave_lds_bytes(B.r), B.r);
-        s += o.qacc[c];
-      }
+      for (int wv = 0; wv < WAVES_PER_WG; wv++) s += red[wv * MAX_COLS + c];
       B.partials[(size_t)blockIdx.x * B.r + c] = s;
     }
   }
@@ -268,6 +265,9 @@ __global__ void __launch_bounds__(256) reduce_q_kernel(const double* __restrict_
 // Launch helpers (called from basis.cpp)
 // ---------------------------------------------------------------------------
 namespace cfs {
+
+// largest r the step kernel supports (q accumulators live in registers)
+int step_max_cols() { return MAX_COLS; }
 
 hipError_t launch_step(const DevBasis& B, int mode, const float* v_in, const double* q_in, double* q_out,
                        float* field_out, const float* d_damp, const void* d_tab, int grid, hipStream_t s) {
