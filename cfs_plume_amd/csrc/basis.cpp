@@ -199,7 +199,7 @@ const HostTables& host_tables() {
 
 struct cfs_basis {
   int device = 0;
-  int nBits = 0, X = 0, Y = 0, Z = 0, Px = 0, Py = 0, Pz = 0, r = 0, nB = 0, nk = 0;
+  int nBits = 0, X = 0, Y = 0, Z = 0, Px = 0, Py = 0, Pz = 0, r = 0, nB = 0, nk = 0, kmin = 0;
   int64_t ntok[3] = {0, 0, 0};
   int64_t nhazards = 0;
   int grid[4] = {0, 0, 0, 0};  // persistent grid per mode (index = mode)
@@ -237,7 +237,6 @@ int build_basis(const uint8_t* const bufs[3], const size_t sizes[3], cfs_basis**
   B->X = f[0].X; B->Y = f[0].Y; B->Z = f[0].Z;
   B->Px = f[0].Px; B->Py = f[0].Py; B->Pz = f[0].Pz;
   B->r = f[0].r; B->nB = f[0].nB;
-  B->nk = 4 * B->nBits + 1;
   const int r = B->r, nB = B->nB;
   const size_t nseg = (size_t)3 * nB * r;
 
@@ -245,7 +244,17 @@ int build_basis(const uint8_t* const bufs[3], const size_t sizes[3], cfs_basis**
   std::vector<uint64_t> blk_base((size_t)3 * nB);
   std::vector<uint32_t> blk_len((size_t)3 * nB);
   std::vector<uint32_t> seg_lk(nseg), seg_off(nseg);
-  std::vector<float> seg_sinv(nseg), seg_vmax(nseg);
+  std::vector<float> seg_sinv(nseg), seg_vmax(nseg), blk_bound((size_t)3 * nB);
+  // damping-table rows actually used: k4 in [kmin, kmax]
+  int kmin = 4 * B->nBits, kmax = 0;
+  for (int k = 0; k < 3; k++)
+    for (size_t i = 0; i < (size_t)nB * r; i++) {
+      const int k4 = (int)(4.0 * f[k].gamma[i]);
+      kmin = std::min(kmin, k4);
+      kmax = std::max(kmax, k4);
+    }
+  B->kmin = kmin;
+  B->nk = kmax - kmin + 1;
   uint64_t pos = 0;
   for (int k = 0; k < 3; k++) {
     B->ntok[k] = (int64_t)f[k].ntok;
@@ -257,7 +266,7 @@ int build_basis(const uint8_t* const bufs[3], const size_t sizes[3], cfs_basis**
         const size_t e2 = (size_t)b + (size_t)nB * c;  // file order
         const size_t si = kb * r + c;                   // device order
         const uint32_t L = (uint32_t)f[k].len[e2];
-        const uint32_t k4 = (uint32_t)(4.0 * f[k].gamma[e2]);
+        const uint32_t k4 = (uint32_t)(4.0 * f[k].gamma[e2]) - (uint32_t)kmin;
         seg_lk[si] = L | (k4 << 16);
         seg_off[si] = (uint32_t)off;
         seg_sinv[si] = (float)(std::ldexp(1.0 / f[k].s[e2], FACTOR_SHIFT));
@@ -304,7 +313,7 @@ int build_basis(const uint8_t* const bufs[3], const size_t sizes[3], cfs_basis**
         const int L = (int)(seg_lk[si] & 0xffffu);
         bool single = false;
         double vmax = 0.0;
-        const char* why = check_segment(stream + seg_off[si], L, dpow.data() + (size_t)(seg_lk[si] >> 16) * NDIAG,
+        const char* why = check_segment(stream + seg_off[si], L, dpow.data() + (size_t)((seg_lk[si] >> 16) + B->kmin) * NDIAG,
                                         dzt, &single, &vmax);
         if (why) {
           bad[kb] = "component " + std::to_string(k) + " block " + std::to_string(b) + " column " +
@@ -320,6 +329,12 @@ int build_basis(const uint8_t* const bufs[3], const size_t sizes[3], cfs_basis**
   });
   for (const auto& m : bad)
     if (!m.empty()) return fail(CFS_ERR_FORMAT, m);
+  // per block stream: sum_c vmax_c / s_c, the bound of |sum_c q_c D_c[j]| / max|q|
+  for (size_t kb = 0; kb < (size_t)3 * nB; kb++) {
+    double acc = 0.0;
+    for (int c = 0; c < r; c++) acc += (double)seg_vmax[kb * r + c] * (double)seg_sinv[kb * r + c];
+    blk_bound[kb] = (float)(acc * (1.0 + 1e-5));
+  }
   std::vector<uint32_t> hz_start((size_t)3 * nB + 1, 0), hz;
   for (size_t kb = 0; kb < (size_t)3 * nB; kb++) {
     hz_start[kb] = (uint32_t)hz.size();
@@ -340,7 +355,7 @@ int build_basis(const uint8_t* const bufs[3], const size_t sizes[3], cfs_basis**
   std::vector<float> damp((size_t)B->nk * NDIAG);
   for (int k = 0; k < B->nk; k++)
     for (int d = 0; d < NDIAG; d++)
-      damp[(size_t)k * NDIAG + d] = (float)std::ldexp(std::pow((double)(1 + d), k / 4.0), -FACTOR_SHIFT);
+      damp[(size_t)k * NDIAG + d] = (float)std::ldexp(std::pow((double)(1 + d), (k + kmin) / 4.0), -FACTOR_SHIFT);
   std::vector<uint8_t> tab(tables_bytes());
   const HostTables& T = host_tables();
   fill_tables(tab.data(), T.dct, T.zig, T.rev, T.dz);
@@ -368,6 +383,7 @@ int build_basis(const uint8_t* const bufs[3], const size_t sizes[3], cfs_basis**
   uint32_t *d_bl = nullptr, *d_lk = nullptr, *d_off = nullptr, *d_err = nullptr, *d_cnt = nullptr;
   uint32_t *d_hzs = nullptr, *d_hz = nullptr;
   float* d_vmax = nullptr;
+  float* d_bound = nullptr;
   float *d_sinv = nullptr, *d_damp = nullptr;
   int32_t* d_order = nullptr;
   double* d_part = nullptr;
@@ -386,6 +402,7 @@ int build_basis(const uint8_t* const bufs[3], const size_t sizes[3], cfs_basis**
   UP(d_off, seg_off.data(), seg_off.size());
   UP(d_sinv, seg_sinv.data(), seg_sinv.size());
   UP(d_vmax, seg_vmax.data(), seg_vmax.size());
+  UP(d_bound, blk_bound.data(), blk_bound.size());
   UP(d_damp, damp.data(), damp.size());
   UP(d_tab, tab.data(), tab.size());
   UP(d_order, order.data(), order.size());
@@ -403,6 +420,7 @@ int build_basis(const uint8_t* const bufs[3], const size_t sizes[3], cfs_basis**
   D.seg_off = d_off;
   D.seg_sinv = d_sinv;
   D.seg_vmax = d_vmax;
+  D.blk_bound = d_bound;
   D.damp = d_damp;
   D.order = d_order;
   D.hz_start = d_hzs;

@@ -12,8 +12,8 @@
 //   * Per segment, in the same block-major order: seg_lk = length | (4*gamma)
 //     << 16 and seg_sinv = (float)(1/s); seg_off = offset inside the stream.
 //   * The damping table (1+d)^(k/4) for d = u+v+w in [0,21] and k = 4*gamma in
-//     [0, 4*nBits] (COMPRESSION_DATA.cpp:25-43 restricted to the 22 distinct
-//     bases) is rounded once from FP64 to FP32.  Table entries are stored times
+//     [kmin, kmax], the range the basis uses (COMPRESSION_DATA.cpp:25-43
+//     restricted to the 22 distinct bases), is rounded once from FP64 to FP32.  Table entries are stored times
 //     2^-FACTOR_SHIFT and 1/s times 2^FACTOR_SHIFT: 22^32 = 9.4e42 exceeds the
 //     FP32 range, the scaled pair does not, and their product is unchanged.
 #pragma once
@@ -40,20 +40,21 @@ struct DevBasis {
   const int32_t* tokens;      // block streams + guard
   const uint64_t* blk_base;   // [3*nB] first token of block stream (k, b)
   const uint32_t* blk_len;    // [3*nB] tokens in block stream (k, b)
-  const uint32_t* seg_lk;     // [3*nB*r] length | k4 << 16, block-major
+  const uint32_t* seg_lk;     // [3*nB*r] length | (k4 - kmin) << 16, block-major
   const uint32_t* seg_off;    // [3*nB*r] offset of segment inside its block stream
   const float* seg_sinv;      // [3*nB*r] 2^FACTOR_SHIFT / s
   const float* seg_vmax;      // [3*nB*r] max |int| * damp^gamma * 2^-FACTOR_SHIFT of the segment
-  const float* damp;          // [nk * 22] (1+d)^(k/4) * 2^-FACTOR_SHIFT in FP32
+  const float* blk_bound;     // [3*nB] sum over the block stream's segments of vmax/s
+  const float* damp;          // [nk * 22] (1+d)^((kmin+k)/4) * 2^-FACTOR_SHIFT in FP32
   const uint32_t* hz_start;   // [3*nB+1] first hazard of block stream (k, b) in hz
   const uint32_t* hz;         // hazard offsets (segment-final singleton equal to the next token)
   const int32_t* order;       // [nB] blocks in descending work order
   uint32_t* err;              // device error bits
-  uint32_t* counter;          // persistent-kernel work counter (left at 0 after each step)
+  uint32_t* counter;          // [2] spare device words (kept zero)
   double* partials;           // [grid][r] per-workgroup FP64 partial q
   int X, Y, Z;                // peeled dims
   int nbx, nby, nbz, nB;      // blocks per axis, total
-  int r, nk;                  // columns, 4*nBits+1
+  int r, nk;                  // columns, rows of the damping table (kmax - kmin + 1)
 };
 
 // arguments of the random-access decode kernel (decode_kernels.hip)

@@ -53,7 +53,7 @@ decode_kernel(DevBasis B, DecodeArgs a, const float* __restrict__ g_damp, const 
   StreamParser<DEC_TPL> parser(B.tokens + aligned, lo, lo + len, nullptr, 0);
   for (int base = 0; base < lo + len; base += 64 * DEC_TPL) {
     Chunk<DEC_TPL> ch;
-    parser.parse(base, ch);
+    parser.template parse<false>(base, ch);
 #pragma unroll
     for (int i = 0; i < DEC_TPL; i++) {
       if (ch.em[i] > 0 && ch.v[i] != 0) {

@@ -34,21 +34,20 @@ constexpr int MAX_COLS = 64 * QREG;
 struct WaveLds {
   int4* prm;     // [r] {22*k4, bits(2^64/s), bits(fixed-point weight of q_c), 0}
   double* ends;  // [rp] running project sum of the block stream at the end of each segment
+  double* qacc;  // [rp] this wave's FP64 project accumulator
   float* vz;     // [APAD] DCT(v) block in zigzag order (padded index)
   int* az;       // [APAD] unproject accumulator, zigzag order (padded index), fixed point
 };
 
 __host__ __device__ inline size_t step_wave_lds_bytes(int r) {
   const size_t rp = (size_t)((r + 1) & ~1);
-  return ((size_t)r * 16 + rp * 8 + 2 * APAD * 4 + 15) / 16 * 16;
+  return ((size_t)r * 16 + rp * 16 + 2 * APAD * 4 + 15) / 16 * 16;
 }
 __host__ __device__ inline size_t step_shared_lds_bytes(int nk) {
-  return ((size_t)nk * NDIAG * 4 + 15) / 16 * 16 + BC * 2 + BC;
+  return ((size_t)nk * NDIAG * 4 + 15) / 16 * 16 + BC;
 }
 __host__ inline size_t step_lds_bytes(int r, int nk) {
-  const size_t per_waves = WAVES_PER_WG * step_wave_lds_bytes(r);
-  const size_t red = WAVES_PER_WG * MAX_COLS * 8;  // end-of-kernel q reduction reuses the wave area
-  return step_shared_lds_bytes(nk) + (per_waves > red ? per_waves : red);
+  return step_shared_lds_bytes(nk) + WAVES_PER_WG * step_wave_lds_bytes(r);
 }
 
 __device__ __forceinline__ WaveLds carve_wave(char* base, int r) {
@@ -57,40 +56,20 @@ __device__ __forceinline__ WaveLds carve_wave(char* base, int r) {
   w.prm = reinterpret_cast<int4*>(base);
   char* p = base + (size_t)r * 16;
   w.ends = reinterpret_cast<double*>(p); p += (size_t)rp * 8;
+  w.qacc = reinterpret_cast<double*>(p); p += (size_t)rp * 8;
   w.vz = reinterpret_cast<float*>(p); p += APAD * 4;
   w.az = reinterpret_cast<int*>(p);
   return w;
 }
 
-// Loads the per-segment parameters of block stream (k, b) into LDS and
-// returns 1/S, the unit of the fixed-point unproject accumulator.
-//
 // Unproject accumulates A[j] = sum_c q_c D_c[j] with integer LDS atomics
-// (gfx950's LDS float atomics are far slower).  Every term is bounded by
-// |q_c| * vmax_c / s_c (vmax_c from the loader), so with
-// S = 2^30 / sum_c |q_c| vmax_c / s_c every partial sum of rounded terms stays
-// below 2^31; the rounding error is below r units of 1/S per entry.
-__device__ float load_block_params(const DevBasis& B, int k, int b, const double* __restrict__ q_in,
-                                   const WaveLds& w, bool unproject) {
-  const int lane = lane_id();
-  const size_t i0 = ((size_t)k * B.nB + b) * B.r;
-  float bound = 0.0f;
-  if (unproject)
-    for (int c = lane; c < B.r; c += WAVE)
-      bound += fabsf((float)(q_in[c] * (double)B.seg_sinv[i0 + c])) * B.seg_vmax[i0 + c];
-  bound = wave_sum(bound) * 1.0001f;
-  const float S = bound > 0.0f ? 1073741824.0f / bound : 0.0f;
-  for (int c = lane; c < B.r; c += WAVE) {
-    const uint32_t lk = B.seg_lk[i0 + c];
-    const float sv = B.seg_sinv[i0 + c];
-    const float qs = unproject ? (float)(q_in[c] * (double)sv) * S : 0.0f;
-    w.prm[c] = make_int4((int)(lk >> 16) * NDIAG, __float_as_int(sv), __float_as_int(qs), 0);
-  }
-  return S > 0.0f ? 1.0f / S : 0.0f;
-}
-
+// (gfx950's LDS float atomics are far slower).  |D_c[j]| <= vmax_c / s_c
+// (vmax_c from the loader), so |A[j]| <= max|q| * blk_bound with
+// blk_bound = sum_c vmax_c / s_c; with S = 2^30 / (max|q| * blk_bound) every
+// partial sum of rounded terms stays below 2^31 and the rounding error is
+// below r units of 1/S per entry.
 template <int MODE>
-__global__ void __launch_bounds__(WAVES_PER_WG * WAVE)
+__global__ void __launch_bounds__(WAVES_PER_WG * WAVE, 3)  // <= 168 VGPRs: three waves per SIMD
 step_kernel(DevBasis B, const float* __restrict__ v_in, const double* __restrict__ q_in,
             float* __restrict__ field_out, const float* __restrict__ g_damp, const Tables* __restrict__ g_tab) {
   extern __shared__ __align__(16) char lds[];
@@ -100,49 +79,78 @@ step_kernel(DevBasis B, const float* __restrict__ v_in, const double* __restrict
   const int nk = B.nk;
   float* damp = reinterpret_cast<float*>(lds);
   const size_t dbytes = ((size_t)nk * NDIAG * 4 + 15) / 16 * 16;
-  uint16_t* zig = reinterpret_cast<uint16_t*>(lds + dbytes);
-  uint8_t* dz = reinterpret_cast<uint8_t*>(lds + dbytes + BC * 2);
+  uint8_t* dz = reinterpret_cast<uint8_t*>(lds + dbytes);
+  const uint16_t* zig = g_tab->zig;  // used only by the per-block DCTs
   const int wave = threadIdx.x / WAVE;
   const int lane = lane_id();
   const WaveLds w = carve_wave(lds + step_shared_lds_bytes(nk) + (size_t)wave * step_wave_lds_bytes(B.r), B.r);
+  const int r = B.r;
 
   for (int i = threadIdx.x; i < nk * NDIAG; i += blockDim.x) damp[i] = g_damp[i];
-  for (int i = threadIdx.x; i < BC; i += blockDim.x) {
-    zig[i] = g_tab->zig[i];
-    dz[i] = g_tab->dz[i];
-  }
+  for (int i = threadIdx.x; i < BC; i += blockDim.x) dz[i] = g_tab->dz[i];
   for (int i = lane; i < APAD; i += WAVE) w.az[i] = 0;
+  for (int c = lane; c < r; c += WAVE) w.qacc[c] = 0.0;
+  // q is the same for every block: keep this lane's columns in registers
+  float qf[QREG];
+  float qmax = 0.0f;
+#pragma unroll
+  for (int m = 0; m < QREG; m++) {
+    const int c = lane + WAVE * m;
+    qf[m] = (UNPROJ && c < r) ? (float)q_in[c] : 0.0f;
+    qmax = fmaxf(qmax, fabsf(qf[m]));
+  }
+  if (UNPROJ) qmax = wave_max(qmax);
   __syncthreads();
   const float* C = g_tab->dct;
 
-  double qacc[QREG];
-#pragma unroll
-  for (int m = 0; m < QREG; m++) qacc[m] = 0.0;
   bool complete = true;
   const int total = 3 * B.nB;
-  const int last_seg = B.r - 1;
-  for (;;) {
-    int item = 0;
-    if (lane == 0) item = (int)atomicAdd(B.counter, 1u);
-    item = uniform(item);
-    if (item >= total) break;
+  const int last_seg = r - 1;
+  const int nwaves = gridDim.x * WAVES_PER_WG;
+  // Static round-robin over items sorted by decreasing work: the three
+  // components of a block go to neighbouring waves and run together.
+  for (int item = blockIdx.x * WAVES_PER_WG + wave; item < total; item += nwaves) {
     const int b = B.order[item / 3];
     const int k = item % 3;
     const size_t kb = (size_t)k * B.nB + b;
-
-    const float inv_scale = load_block_params(B, k, b, q_in, w, UNPROJ);
+    const size_t i0 = kb * r;
+    const int len = (int)B.blk_len[kb];
+    const uint64_t tbase = B.blk_base[kb];
+    const uint32_t h0 = B.hz_start[kb], h1 = B.hz_start[kb + 1];
+    // per-segment parameters: all loads first, one round trip
+    uint32_t lk[QREG];
+    float sv[QREG];
+#pragma unroll
+    for (int m = 0; m < QREG; m++) {
+      const int c = lane + WAVE * m;
+      lk[m] = c < r ? B.seg_lk[i0 + c] : 0u;
+      sv[m] = c < r ? B.seg_sinv[i0 + c] : 0.0f;
+    }
+    float inv_scale = 0.0f, S = 0.0f;
+    if (UNPROJ) {
+      const float bound = B.blk_bound[kb] * qmax * 1.0001f;
+      S = bound > 0.0f ? 1073741824.0f / bound : 0.0f;
+      inv_scale = S > 0.0f ? 1.0f / S : 0.0f;
+    }
+#pragma unroll
+    for (int m = 0; m < QREG; m++) {
+      const int c = lane + WAVE * m;
+      if (c < r)
+        w.prm[c] = make_int4((int)(lk[m] >> 16) * NDIAG, __float_as_int(sv[m]), __float_as_int(qf[m] * sv[m] * S), 0);
+    }
     if (PROJ) forward_dct_block(v_in, B, k, b, C, zig, reinterpret_cast<float*>(w.az), w.vz);
     if (PROJ && UNPROJ)
       for (int i = lane; i < APAD; i += WAVE) w.az[i] = 0;  // az served as DCT scratch
     wave_lds_sync();
 
-    const int len = (int)B.blk_len[kb];
-    const uint32_t h0 = B.hz_start[kb];
-    StreamParser<NT> parser(B.tokens + B.blk_base[kb], 0, len, B.hz + h0, (int)(B.hz_start[kb + 1] - h0));
+    StreamParser<NT> parser(B.tokens + tbase, 0, len, B.hz + h0, (int)(h1 - h0));
     double run = 0.0;  // project sum of the chunks before this one (wave-uniform)
     for (int base = 0; base < len; base += CH) {
       Chunk<NT> ch;
-      parser.parse(base, ch);
+      if (parser.plain(base))
+        parser.template parse<true>(base, ch);
+      else
+        parser.template parse<false>(base, ch);
       // per-token segment parameters and table lookups for the first value
       int4 pr[NT];
       int jj[NT], dd[NT];
@@ -171,14 +179,14 @@ step_kernel(DevBasis B, const float* __restrict__ v_in, const double* __restrict
 #pragma unroll
         for (int i = 0; i < NT; i++) {
           if (nz[i] && ch.em[i] > 1) {
-            const float sv = __int_as_float(pr[i].y);
+            const float svi = __int_as_float(pr[i].y);
             const float qs = __int_as_float(pr[i].z);
             const float v0 = (float)ch.v[i];
             for (int m = 1; m < ch.em[i]; m++) {
               const int j = (ch.P[i] + m) & (BC - 1);
               const int jp = padj(j);
               const float vf = v0 * damp[pr[i].x + dz[j]];
-              if (PROJ) prod[i] += vf * w.vz[jp] * sv;
+              if (PROJ) prod[i] += vf * w.vz[jp] * svi;
               if (UNPROJ) atomicAdd(&w.az[jp], (int)(vf * qs));
             }
           }
@@ -202,43 +210,24 @@ step_kernel(DevBasis B, const float* __restrict__ v_in, const double* __restrict
         run += (double)readlanef(incl, WAVE - 1);
       }
     }
-    complete = complete && parser.complete(B.r);
+    complete = complete && parser.complete(r);
     wave_lds_sync();
 
-    if (PROJ) {
-#pragma unroll
-      for (int m = 0; m < QREG; m++) {
-        const int c = lane + WAVE * m;
-        if (c < B.r) qacc[m] += w.ends[c] - (c > 0 ? w.ends[c - 1] : 0.0);
-      }
-    }
+    if (PROJ)
+      for (int c = lane; c < r; c += WAVE) w.qacc[c] += w.ends[c] - (c > 0 ? w.ends[c - 1] : 0.0);
     if (UNPROJ) inverse_dct_block(field_out, B, k, b, C, zig, w.az, inv_scale, w.vz);
     wave_lds_sync();
   }
   if (!complete && lane == 0) atomicOr(B.err, (uint32_t)ERR_INCOMPLETE);
 
   if (PROJ) {
-    // combine the waves' FP64 accumulators in a fixed order (through LDS)
+    // combine the waves' FP64 accumulators in a fixed order
     __syncthreads();
-    double* red = reinterpret_cast<double*>(lds + step_shared_lds_bytes(nk));  // [WAVES_PER_WG][MAX_COLS]
-#pragma unroll
-    for (int m = 0; m < QREG; m++) red[wave * MAX_COLS + lane + WAVE * m] = qacc[m];
-    __syncthreads();
-    for (int c = threadIdx.x; c < B.r; c += blockDim.x) {
+    for (int c = threadIdx.x; c < r; c += blockDim.x) {
       double s = 0.0;
-      for (int wv = 0; wv < WAVES_PER_WG; wv++) s += red[wv * MAX_COLS + c];
-      B.partials[(size_t)blockIdx.x * B.r + c] = s;
-    }
-  }
-  // The last workgroup to finish resets the work counter for the next launch.
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    const unsigned done = atomicAdd(B.counter + 1, 1u);
-    if (done == gridDim.x - 1) {
-      B.counter[0] = 0u;
-      B.counter[1] = 0u;
-      __threadfence();
+      for (int wv = 0; wv < WAVES_PER_WG; wv++)
+        s += carve_wave(lds + step_shared_lds_bytes(nk) + (size_t)wv * step_wave_lds_bytes(r), r).qacc[c];
+      B.partials[(size_t)blockIdx.x * r + c] = s;
     }
   }
 }

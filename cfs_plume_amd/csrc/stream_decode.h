@@ -53,7 +53,14 @@ class StreamParser {
     next_hz_ = nhz_ > 0 ? (int)hz_[0] : 0x7fffffff;
   }
 
+  // True if the chunk at `base` lies inside [lo, hi - 1) and holds no hazard,
+  // so parse<true> may skip the range and hazard tests (wave-uniform).
+  __device__ __forceinline__ bool plain(int base) const {
+    return base >= lo_ && base + CH < len_ && next_hz_ >= base + CH;
+  }
+
   // Parses the chunk starting at token `base` (a multiple of 64*NT).
+  template <bool PLAIN>
   __device__ __forceinline__ void parse(int base, Chunk<NT>& ch) {
     // Two chunks are in flight: this one and the next (whose first two tokens
     // complete lane 63's lookahead).  The token array ends with a guard of
@@ -79,7 +86,7 @@ class StreamParser {
 
     // segment-final singletons equal to the next token (rare)
     uint32_t hzm = 0;
-    while (next_hz_ < base + CH) {
+    while (!PLAIN && next_hz_ < base + CH) {
       const int o = next_hz_ - base;
       if (lane_ == o / NT) hzm |= 1u << (o % NT);
       hz_i_++;
@@ -91,9 +98,14 @@ class StreamParser {
     bool eq[NT];
 #pragma unroll
     for (int i = 0; i < NT; i++) {
-      const bool valid = tid0 + i >= lo_ && tid0 + i < len_;
-      eq[i] = !((hzm >> i) & 1u) && (tid0 + i + 1 < len_) && t[i] == t[i + 1];
-      f[i] = !valid ? FSM_ID : (eq[i] ? FSM_EQ : FSM_NE);
+      if (PLAIN) {
+        eq[i] = t[i] == t[i + 1];
+        f[i] = eq[i] ? FSM_EQ : FSM_NE;
+      } else {
+        const bool valid = tid0 + i >= lo_ && tid0 + i < len_;
+        eq[i] = !((hzm >> i) & 1u) && (tid0 + i + 1 < len_) && t[i] == t[i + 1];
+        f[i] = !valid ? FSM_ID : (eq[i] ? FSM_EQ : FSM_NE);
+      }
     }
     uint32_t F = f[0];
 #pragma unroll
@@ -110,7 +122,7 @@ class StreamParser {
     int lane_sum = 0, pre[NT];
 #pragma unroll
     for (int i = 0; i < NT; i++) {
-      const bool isv = (st == 0u) && f[i] != FSM_ID;
+      const bool isv = (st == 0u) && (PLAIN || f[i] != FSM_ID);
       ch.em[i] = isv ? (eq[i] ? t[i + 2] : 1) : 0;
       ch.v[i] = t[i];
       pre[i] = lane_sum;

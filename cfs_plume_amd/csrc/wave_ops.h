@@ -58,25 +58,32 @@ __device__ __forceinline__ uint32_t wave_scan_fsm(uint32_t f) {
   return f;
 }
 
+// DPP move whose out-of-range / masked lanes read 0 (bound_ctrl), the form
+// the compiler folds into the consuming VALU instruction.
+template <int CTRL, int ROW_MASK = 0xf, int BANK_MASK = 0xf>
+__device__ __forceinline__ int dpp0(int src) {
+  return __builtin_amdgcn_update_dpp(0, src, CTRL, ROW_MASK, BANK_MASK, true);
+}
+
 // Inclusive integer prefix sum across the wave.
 __device__ __forceinline__ int wave_scan_add(int x) {
-  x += (int)dpp<DPP_ROW_SHR1>(0u, (uint32_t)x);
-  x += (int)dpp<DPP_ROW_SHR2>(0u, (uint32_t)x);
-  x += (int)dpp<DPP_ROW_SHR4>(0u, (uint32_t)x);
-  x += (int)dpp<DPP_ROW_SHR8>(0u, (uint32_t)x);
-  x += (int)dpp<DPP_ROW_BCAST15, 0xa>(0u, (uint32_t)x);
-  x += (int)dpp<DPP_ROW_BCAST31, 0xc>(0u, (uint32_t)x);
+  x += dpp0<DPP_ROW_SHR1>(x);
+  x += dpp0<DPP_ROW_SHR2>(x);
+  x += dpp0<DPP_ROW_SHR4>(x);
+  x += dpp0<DPP_ROW_SHR8>(x);
+  x += dpp0<DPP_ROW_BCAST15, 0xa>(x);
+  x += dpp0<DPP_ROW_BCAST31, 0xc>(x);
   return x;
 }
 
 // Inclusive float prefix sum across the wave.
 __device__ __forceinline__ float wave_scan_addf(float x) {
-  x += dppf<DPP_ROW_SHR1>(0.0f, x);
-  x += dppf<DPP_ROW_SHR2>(0.0f, x);
-  x += dppf<DPP_ROW_SHR4>(0.0f, x);
-  x += dppf<DPP_ROW_SHR8>(0.0f, x);
-  x += dppf<DPP_ROW_BCAST15, 0xa>(0.0f, x);
-  x += dppf<DPP_ROW_BCAST31, 0xc>(0.0f, x);
+  x += __int_as_float(dpp0<DPP_ROW_SHR1>(__float_as_int(x)));
+  x += __int_as_float(dpp0<DPP_ROW_SHR2>(__float_as_int(x)));
+  x += __int_as_float(dpp0<DPP_ROW_SHR4>(__float_as_int(x)));
+  x += __int_as_float(dpp0<DPP_ROW_SHR8>(__float_as_int(x)));
+  x += __int_as_float(dpp0<DPP_ROW_BCAST15, 0xa>(__float_as_int(x)));
+  x += __int_as_float(dpp0<DPP_ROW_BCAST31, 0xc>(__float_as_int(x)));
   return x;
 }
 
@@ -98,6 +105,15 @@ __device__ __forceinline__ float wave_sum(float x) {
   x += __shfl_xor(x, 4);
   x += __shfl_xor(x, 2);
   x += __shfl_xor(x, 1);
+  return x;
+}
+__device__ __forceinline__ float wave_max(float x) {
+  x = fmaxf(x, __shfl_xor(x, 32));
+  x = fmaxf(x, __shfl_xor(x, 16));
+  x = fmaxf(x, __shfl_xor(x, 8));
+  x = fmaxf(x, __shfl_xor(x, 4));
+  x = fmaxf(x, __shfl_xor(x, 2));
+  x = fmaxf(x, __shfl_xor(x, 1));
   return x;
 }
 __device__ __forceinline__ double wave_sum(double x) {
