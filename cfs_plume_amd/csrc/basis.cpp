@@ -123,7 +123,8 @@ void parallel_for(size_t n, const std::function<void(size_t, size_t)>& fn) {
 // Also returns vmax = max |int| * (1+u+v+w)^gamma over the segment, the bound
 // of its dequantised values times s (damp_row: the 22 powers for its gamma).
 const char* check_segment(const int32_t* t, int len, const double* damp_row, const uint8_t* dz,
-                          bool* ends_in_singleton, double* vmax) {
+                          bool* ends_in_singleton, double* vmax, std::vector<uint32_t>* long_runs,
+                          uint32_t seg_off) {
   int i = 0, j = 0;
   bool single = false;
   double m = 0.0;
@@ -135,8 +136,10 @@ const char* check_segment(const int32_t* t, int len, const double* damp_row, con
       const int L = t[i + 2];
       if (L < 2 || L > BC) return "run length outside [2, 512]";
       if (j + L > BC) return "decodes to more than 512 values";
-      if (a > 0)
+      if (a > 0) {
         for (int q = j; q < j + L; q++) m = std::max(m, a * damp_row[dz[q]]);
+        if (L >= 4) long_runs->push_back(seg_off + (uint32_t)i + 2);  // the length token
+      }
       j += L;
       i += 3;
       single = false;
@@ -202,6 +205,7 @@ struct cfs_basis {
   int nBits = 0, X = 0, Y = 0, Z = 0, Px = 0, Py = 0, Pz = 0, r = 0, nB = 0, nk = 0, kmin = 0;
   int64_t ntok[3] = {0, 0, 0};
   int64_t nhazards = 0;
+  int64_t nlong = 0;  // nonzero runs with L >= 4
   int grid[4] = {0, 0, 0, 0};  // persistent grid per mode (index = mode)
   size_t device_bytes = 0;
   DevBasis dev{};
@@ -298,7 +302,7 @@ int build_basis(const uint8_t* const bufs[3], const size_t sizes[3], cfs_basis**
   });
   // Check every segment; record the "hazard" boundaries of each block stream:
   // a segment ending in a singleton equal to the next segment's first token.
-  std::vector<std::vector<uint32_t>> hz_lists((size_t)3 * nB);
+  std::vector<std::vector<uint32_t>> hz_lists((size_t)3 * nB), lr_lists((size_t)3 * nB);
   std::vector<std::string> bad((size_t)3 * nB);
   std::vector<double> dpow((size_t)(4 * B->nBits + 1) * NDIAG);
   for (int k = 0; k <= 4 * B->nBits; k++)
@@ -314,7 +318,7 @@ int build_basis(const uint8_t* const bufs[3], const size_t sizes[3], cfs_basis**
         bool single = false;
         double vmax = 0.0;
         const char* why = check_segment(stream + seg_off[si], L, dpow.data() + (size_t)((seg_lk[si] >> 16) + B->kmin) * NDIAG,
-                                        dzt, &single, &vmax);
+                                        dzt, &single, &vmax, &lr_lists[kb], seg_off[si]);
         if (why) {
           bad[kb] = "component " + std::to_string(k) + " block " + std::to_string(b) + " column " +
                     std::to_string(c) + ": segment " + why;
@@ -342,6 +346,15 @@ int build_basis(const uint8_t* const bufs[3], const size_t sizes[3], cfs_basis**
   }
   hz_start[(size_t)3 * nB] = (uint32_t)hz.size();
   B->nhazards = (int64_t)hz.size();
+  // length tokens of nonzero runs longer than 3 (their values past P+2 are
+  // added outside the fused step's per-token pass)
+  std::vector<uint32_t> lr_start((size_t)3 * nB + 1, 0), lr;
+  for (size_t kb = 0; kb < (size_t)3 * nB; kb++) {
+    lr_start[kb] = (uint32_t)lr.size();
+    lr.insert(lr.end(), lr_lists[kb].begin(), lr_lists[kb].end());
+  }
+  lr_start[(size_t)3 * nB] = (uint32_t)lr.size();
+  B->nlong = (int64_t)lr.size();
 
   // work order: blocks with the longest streams first
   std::vector<int32_t> order(nB);
@@ -381,7 +394,7 @@ int build_basis(const uint8_t* const bufs[3], const size_t sizes[3], cfs_basis**
   int32_t* d_tok = nullptr;
   uint64_t* d_bb = nullptr;
   uint32_t *d_bl = nullptr, *d_lk = nullptr, *d_off = nullptr, *d_err = nullptr, *d_cnt = nullptr;
-  uint32_t *d_hzs = nullptr, *d_hz = nullptr;
+  uint32_t *d_hzs = nullptr, *d_hz = nullptr, *d_lrs = nullptr, *d_lr = nullptr;
   float* d_vmax = nullptr;
   float* d_bound = nullptr;
   float *d_sinv = nullptr, *d_damp = nullptr;
@@ -408,6 +421,8 @@ int build_basis(const uint8_t* const bufs[3], const size_t sizes[3], cfs_basis**
   UP(d_order, order.data(), order.size());
   UP(d_hzs, hz_start.data(), hz_start.size());
   UP(d_hz, hz.data(), hz.size());
+  UP(d_lrs, lr_start.data(), lr_start.size());
+  UP(d_lr, lr.data(), lr.size());
   const uint32_t zeros[4] = {0, 0, 0, 0};
   UP(d_err, zeros, 1);
   UP(d_cnt, zeros, 2);
@@ -425,6 +440,8 @@ int build_basis(const uint8_t* const bufs[3], const size_t sizes[3], cfs_basis**
   D.order = d_order;
   D.hz_start = d_hzs;
   D.hz = d_hz;
+  D.lr_start = d_lrs;
+  D.lr = d_lr;
   D.err = d_err;
   D.counter = d_cnt;
   D.partials = d_part;

@@ -11,9 +11,6 @@
 
 namespace cfs {
 
-constexpr int APAD = BC + BC / 32;  // padded accumulator length (1 word per 32)
-__device__ __forceinline__ int padj(int j) { return j + (j >> 5); }
-
 // Small read-only tables, shared by every basis on the device.
 struct Tables {
   float dct[64];        // C[k][n] = a_k cos(pi (2n+1) k / 16), row k
@@ -79,7 +76,7 @@ __device__ void forward_dct_block(const float* __restrict__ v, const DevBasis& B
       float s = 0.0f;
 #pragma unroll
       for (int z = 0; z < BS; z++) s += C[w * BS + z] * a[z];
-      vz[padj(zig[u + 8 * vv + 64 * w])] = s;
+      vz[zig[u + 8 * vv + 64 * w]] = s;
     }
   }
   wave_lds_sync();
@@ -98,7 +95,7 @@ __device__ void inverse_dct_block(float* __restrict__ field, const DevBasis& B, 
     const int u = lane & 7, vv = lane >> 3;
     float a[BS];
 #pragma unroll
-    for (int w = 0; w < BS; w++) a[w] = (float)az[padj(zig[u + 8 * vv + 64 * w])] * unit;
+    for (int w = 0; w < BS; w++) a[w] = (float)az[zig[u + 8 * vv + 64 * w]] * unit;
 #pragma unroll
     for (int z = 0; z < BS; z++) {
       float s = 0.0f;
@@ -108,7 +105,7 @@ __device__ void inverse_dct_block(float* __restrict__ field, const DevBasis& B, 
     }
   }
   wave_lds_sync();
-  for (int i = lane; i < APAD; i += WAVE) az[i] = 0;
+  for (int i = lane; i < BC; i += WAVE) az[i] = 0;
   // pass along y: lane = (u, z)
   {
     const int u = lane & 7, z = lane >> 3;

@@ -48,6 +48,8 @@ struct DevBasis {
   const float* damp;          // [nk * 22] (1+d)^((kmin+k)/4) * 2^-FACTOR_SHIFT in FP32
   const uint32_t* hz_start;   // [3*nB+1] first hazard of block stream (k, b) in hz
   const uint32_t* hz;         // hazard offsets (segment-final singleton equal to the next token)
+  const uint32_t* lr_start;   // [3*nB+1] first long run of block stream (k, b) in lr
+  const uint32_t* lr;         // offsets of the length tokens of nonzero runs with L >= 4
   const int32_t* order;       // [nB] blocks in descending work order
   uint32_t* err;              // device error bits
   uint32_t* counter;          // [2] spare device words (kept zero)

@@ -28,20 +28,21 @@ enum : int { MODE_PROJECT = 1, MODE_UNPROJECT = 2, MODE_BOTH = 3 };
 constexpr int WAVES_PER_WG = 4;
 constexpr int NT = STEP_TPL;
 constexpr int CH = 64 * NT;
-constexpr int QREG = 4;  // FP64 q accumulators per lane: columns lane + 64*m, r <= 256
+constexpr int QREG = 4;  // segment parameters per lane: columns lane + 64*m, r <= 256
 constexpr int MAX_COLS = 64 * QREG;
+constexpr int NO_END = 0x7fffffff;
 
 struct WaveLds {
-  int4* prm;     // [r] {22*k4, bits(2^64/s), bits(fixed-point weight of q_c), 0}
-  double* ends;  // [rp] running project sum of the block stream at the end of each segment
-  double* qacc;  // [rp] this wave's FP64 project accumulator
-  float* vz;     // [APAD] DCT(v) block in zigzag order (padded index)
-  int* az;       // [APAD] unproject accumulator, zigzag order (padded index), fixed point
+  int4* prm;     // [r+1] {damp row byte offset, bits(2^64/s), bits(fixed-point weight of q_c), end offset};
+                 // entry r is {0, 0, 0, NO_END} (tokens past the stream's end index it)
+  double* qacc;  // [r] this wave's FP64 project accumulator
+  float* vz;     // [512] DCT(v) block in zigzag order
+  int* az;       // [512] unproject accumulator in zigzag order, fixed point
+  float* pfx;    // [512] project prefix sums of the current chunk; DCT scratch
 };
 
 __host__ __device__ inline size_t step_wave_lds_bytes(int r) {
-  const size_t rp = (size_t)((r + 1) & ~1);
-  return ((size_t)r * 16 + rp * 16 + 2 * APAD * 4 + 15) / 16 * 16;
+  return ((size_t)(r + 1) * 16 + (size_t)r * 8 + 3 * BC * 4 + 15) / 16 * 16;
 }
 __host__ __device__ inline size_t step_shared_lds_bytes(int nk) {
   return ((size_t)nk * NDIAG * 4 + 15) / 16 * 16 + BC;
@@ -51,15 +52,25 @@ __host__ inline size_t step_lds_bytes(int r, int nk) {
 }
 
 __device__ __forceinline__ WaveLds carve_wave(char* base, int r) {
-  const int rp = (r + 1) & ~1;
   WaveLds w;
   w.prm = reinterpret_cast<int4*>(base);
-  char* p = base + (size_t)r * 16;
-  w.ends = reinterpret_cast<double*>(p); p += (size_t)rp * 8;
-  w.qacc = reinterpret_cast<double*>(p); p += (size_t)rp * 8;
-  w.vz = reinterpret_cast<float*>(p); p += APAD * 4;
-  w.az = reinterpret_cast<int*>(p);
+  char* p = base + (size_t)(r + 1) * 16;
+  w.vz = reinterpret_cast<float*>(p); p += BC * 4;
+  w.az = reinterpret_cast<int*>(p); p += BC * 4;
+  w.pfx = reinterpret_cast<float*>(p); p += BC * 4;
+  w.qacc = reinterpret_cast<double*>(p);
   return w;
+}
+
+// One value of the stream at output position pos: dequantise (int * damp *
+// (1/s), in the scaled units of the damping table) and feed both operators.
+template <bool PROJ, bool UNPROJ>
+__device__ __forceinline__ float step_value(const char* lds, const uint8_t* dz4, const WaveLds& w, const int4& pr,
+                                            int val, int j) {
+  const float dm = *reinterpret_cast<const float*>(lds + (pr.x + (int)dz4[j]));
+  const float vf = (float)val * dm;
+  if (UNPROJ) atomicAdd(&w.az[j], (int)(vf * __int_as_float(pr.z)));
+  return PROJ ? vf * w.vz[j] * __int_as_float(pr.y) : 0.0f;
 }
 
 // Unproject accumulates A[j] = sum_c q_c D_c[j] with integer LDS atomics
@@ -68,6 +79,10 @@ __device__ __forceinline__ WaveLds carve_wave(char* base, int r) {
 // blk_bound = sum_c vmax_c / s_c; with S = 2^30 / (max|q| * blk_bound) every
 // partial sum of rounded terms stays below 2^31 and the rounding error is
 // below r units of 1/S per entry.
+//
+// Project keeps a running FP64 sum over the block stream (FP32 within a
+// chunk); the sum at the last token of segment c minus the sum at the last
+// token of segment c-1 is segment c's dot product, added to q_c.
 template <int MODE>
 __global__ void __launch_bounds__(WAVES_PER_WG * WAVE, 3)  // <= 168 VGPRs: three waves per SIMD
 step_kernel(DevBasis B, const float* __restrict__ v_in, const double* __restrict__ q_in,
@@ -79,17 +94,18 @@ step_kernel(DevBasis B, const float* __restrict__ v_in, const double* __restrict
   const int nk = B.nk;
   float* damp = reinterpret_cast<float*>(lds);
   const size_t dbytes = ((size_t)nk * NDIAG * 4 + 15) / 16 * 16;
-  uint8_t* dz = reinterpret_cast<uint8_t*>(lds + dbytes);
+  uint8_t* dz4 = reinterpret_cast<uint8_t*>(lds + dbytes);
   const uint16_t* zig = g_tab->zig;  // used only by the per-block DCTs
-  const int wave = threadIdx.x / WAVE;
+  const int wave = uniform((int)(threadIdx.x / WAVE));
   const int lane = lane_id();
   const WaveLds w = carve_wave(lds + step_shared_lds_bytes(nk) + (size_t)wave * step_wave_lds_bytes(B.r), B.r);
   const int r = B.r;
 
   for (int i = threadIdx.x; i < nk * NDIAG; i += blockDim.x) damp[i] = g_damp[i];
-  for (int i = threadIdx.x; i < BC; i += blockDim.x) dz[i] = g_tab->dz[i];
-  for (int i = lane; i < APAD; i += WAVE) w.az[i] = 0;
+  for (int i = threadIdx.x; i < BC; i += blockDim.x) dz4[i] = (uint8_t)(4 * g_tab->dz[i]);
+  for (int i = lane; i < BC; i += WAVE) w.az[i] = 0;
   for (int c = lane; c < r; c += WAVE) w.qacc[c] = 0.0;
+  if (lane == 0) w.prm[r] = make_int4(0, 0, 0, NO_END);
   // q is the same for every block: keep this lane's columns in registers
   float qf[QREG];
   float qmax = 0.0f;
@@ -105,7 +121,6 @@ step_kernel(DevBasis B, const float* __restrict__ v_in, const double* __restrict
 
   bool complete = true;
   const int total = 3 * B.nB;
-  const int last_seg = r - 1;
   const int nwaves = gridDim.x * WAVES_PER_WG;
   // Static round-robin over items sorted by decreasing work: the three
   // components of a block go to neighbouring waves and run together.
@@ -117,6 +132,7 @@ step_kernel(DevBasis B, const float* __restrict__ v_in, const double* __restrict
     const int len = (int)B.blk_len[kb];
     const uint64_t tbase = B.blk_base[kb];
     const uint32_t h0 = B.hz_start[kb], h1 = B.hz_start[kb + 1];
+    const uint32_t l0 = B.lr_start[kb], l1 = B.lr_start[kb + 1];
     // per-segment parameters: all loads first, one round trip
     uint32_t lk[QREG];
     float sv[QREG];
@@ -132,90 +148,100 @@ step_kernel(DevBasis B, const float* __restrict__ v_in, const double* __restrict
       S = bound > 0.0f ? 1073741824.0f / bound : 0.0f;
       inv_scale = S > 0.0f ? 1.0f / S : 0.0f;
     }
+    // segment end offsets: inclusive prefix sum of the lengths (segments are
+    // back to back in the block stream)
+    int carry = 0;
 #pragma unroll
     for (int m = 0; m < QREG; m++) {
       const int c = lane + WAVE * m;
+      const int L = (int)(lk[m] & 0xffffu);
+      const int incl = wave_scan_add(L) + carry;
+      carry = readlane(incl, WAVE - 1);
       if (c < r)
-        w.prm[c] = make_int4((int)(lk[m] >> 16) * NDIAG, __float_as_int(sv[m]), __float_as_int(qf[m] * sv[m] * S), 0);
+        w.prm[c] = make_int4((int)(lk[m] >> 16) * (NDIAG * 4), __float_as_int(sv[m]), __float_as_int(qf[m] * sv[m] * S),
+                             incl);
     }
-    if (PROJ) forward_dct_block(v_in, B, k, b, C, zig, reinterpret_cast<float*>(w.az), w.vz);
-    if (PROJ && UNPROJ)
-      for (int i = lane; i < APAD; i += WAVE) w.az[i] = 0;  // az served as DCT scratch
+    if (PROJ) forward_dct_block(v_in, B, k, b, C, zig, w.pfx, w.vz);
     wave_lds_sync();
 
     StreamParser<NT> parser(B.tokens + tbase, 0, len, B.hz + h0, (int)(h1 - h0));
-    double run = 0.0;  // project sum of the chunks before this one (wave-uniform)
+    int next_lr = l0 < l1 ? (int)B.lr[l0] : NO_END;
+    uint32_t li = l0;
+    double run = 0.0;       // project sum of the chunks before this one (wave-uniform)
+    double last_end = 0.0;  // running sum at the end of the last closed segment
+    int cur = 0;            // first segment not yet closed
     for (int base = 0; base < len; base += CH) {
-      Chunk<NT> ch;
+      Slots<NT> sl;
       if (parser.plain(base))
-        parser.template parse<true>(base, ch);
+        parser.template parse_slots<true>(base, sl);
       else
-        parser.template parse<false>(base, ch);
-      // per-token segment parameters and table lookups for the first value
-      int4 pr[NT];
-      int jj[NT], dd[NT];
-      bool nz[NT];
+        parser.template parse_slots<false>(base, sl);
+      float acc = 0.0f, pre[NT];
 #pragma unroll
       for (int i = 0; i < NT; i++) {
-        nz[i] = ch.em[i] > 0 && ch.v[i] != 0;
-        pr[i] = w.prm[min(ch.P[i] >> 9, last_seg)];
-        const int j = ch.P[i] & (BC - 1);
-        jj[i] = padj(j);
-        dd[i] = dz[j];
+        const int4 pr = w.prm[sl.pos[i] >> 9];
+        acc += step_value<PROJ, UNPROJ>(lds, dz4, w, pr, sl.val[i], sl.pos[i] & (BC - 1));
+        pre[i] = acc;
       }
-      float prod[NT];
-      bool more = false;
-#pragma unroll
-      for (int i = 0; i < NT; i++) {
-        const float dm = damp[pr[i].x + dd[i]];
-        const float vzv = PROJ ? w.vz[jj[i]] : 0.0f;
-        const float vf = nz[i] ? (float)ch.v[i] * dm : 0.0f;
-        prod[i] = vf * vzv * __int_as_float(pr[i].y);  // value * DCT(v), value in units of 1/s
-        if (UNPROJ && nz[i]) atomicAdd(&w.az[jj[i]], (int)(vf * __int_as_float(pr[i].z)));
-        more |= nz[i] && ch.em[i] > 1;
-      }
-      // the remaining values of nonzero runs (rare: runs are mostly zeros)
-      if (__ballot(more)) {
-#pragma unroll
-        for (int i = 0; i < NT; i++) {
-          if (nz[i] && ch.em[i] > 1) {
-            const float svi = __int_as_float(pr[i].y);
-            const float qs = __int_as_float(pr[i].z);
-            const float v0 = (float)ch.v[i];
-            for (int m = 1; m < ch.em[i]; m++) {
-              const int j = (ch.P[i] + m) & (BC - 1);
-              const int jp = padj(j);
-              const float vf = v0 * damp[pr[i].x + dz[j]];
-              if (PROJ) prod[i] += vf * w.vz[jp] * svi;
-              if (UNPROJ) atomicAdd(&w.az[jp], (int)(vf * qs));
-            }
-          }
-        }
-      }
-      if (PROJ) {
-        // Running sum over the stream; the token that closes a segment (its
-        // emission ends on a multiple of 512) records the sum at that point.
-        float pre[NT], acc = 0.0f;
-#pragma unroll
-        for (int i = 0; i < NT; i++) {
-          acc += prod[i];
-          pre[i] = acc;
-        }
-        const float incl = wave_scan_addf(acc);
-        const float excl = incl - acc;
+      // values P+3 .. P+L-1 of nonzero runs with L >= 4 (listed at load)
+      while (next_lr < base + CH) {
+        const int o = next_lr - base;
+        const int ol = o / NT, os = o % NT;
+        int pv = 0;
 #pragma unroll
         for (int i = 0; i < NT; i++)
-          if (ch.em[i] > 0 && ((ch.P[i] + ch.em[i]) & (BC - 1)) == 0)
-            w.ends[min(ch.P[i] >> 9, last_seg)] = run + (double)(excl + pre[i]);
+          if (i == os) pv = sl.pos[i];
+        pv = readlane(pv, ol);                               // position of the value at P+2
+        const int Lv = B.tokens[tbase + next_lr];            // run length
+        const int vv = B.tokens[tbase + next_lr - 1];        // run value (the DUP token)
+        const int4 pr = w.prm[pv >> 9];
+        float ex = 0.0f;
+        for (int m = lane; m < Lv - 3; m += WAVE)
+          ex += step_value<PROJ, UNPROJ>(lds, dz4, w, pr, vv, (pv + 1 + m) & (BC - 1));
+        if (PROJ) {
+          ex = wave_sum(ex);
+          if (lane == ol) {
+            acc += ex;
+#pragma unroll
+            for (int i = 0; i < NT; i++)
+              if (i >= os) pre[i] += ex;
+          }
+        }
+        li++;
+        next_lr = li < l1 ? (int)B.lr[li] : NO_END;
+      }
+      if (PROJ) {
+        const float incl = wave_scan_addf(acc);
+        const float excl = incl - acc;
+        float4* pf = reinterpret_cast<float4*>(w.pfx + NT * lane);
+#pragma unroll
+        for (int q = 0; q < NT / 4; q++)
+          pf[q] = make_float4(excl + pre[4 * q], excl + pre[4 * q + 1], excl + pre[4 * q + 2], excl + pre[4 * q + 3]);
+        wave_lds_sync();
+        // segments cur, cur+1, ... whose last token lies in this chunk
+        for (;;) {
+          const int c = cur + lane;
+          const int e = w.prm[min(c, r)].w;
+          const bool closes = e <= base + CH;
+          const int pi = closes ? e - 1 - base : 0;
+          const double endv = closes ? run + (double)w.pfx[pi] : 0.0;
+          const double prevv = __shfl_up(endv, 1);
+          const double d = endv - (lane == 0 ? last_end : prevv);
+          if (closes) w.qacc[c] += d;
+          const uint64_t mask = __ballot(closes);
+          const int n = __popcll(mask);
+          if (n > 0) last_end = __shfl(endv, n - 1);
+          cur += n;
+          if (n < WAVE) break;
+        }
         run += (double)readlanef(incl, WAVE - 1);
+        wave_lds_sync();
       }
     }
     complete = complete && parser.complete(r);
     wave_lds_sync();
 
-    if (PROJ)
-      for (int c = lane; c < r; c += WAVE) w.qacc[c] += w.ends[c] - (c > 0 ? w.ends[c - 1] : 0.0);
-    if (UNPROJ) inverse_dct_block(field_out, B, k, b, C, zig, w.az, inv_scale, w.vz);
+    if (UNPROJ) inverse_dct_block(field_out, B, k, b, C, zig, w.az, inv_scale, w.pfx);
     wave_lds_sync();
   }
   if (!complete && lane == 0) atomicOr(B.err, (uint32_t)ERR_INCOMPLETE);

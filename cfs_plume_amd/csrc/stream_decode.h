@@ -31,6 +31,18 @@ struct Chunk {
   int em[NT];    // values emitted by the token (0 for DUP/LEN/out of range)
 };
 
+// Per-token view used by the fused step: every token contributes at most one
+// value inline.  A run [v, v, L] at output position P is split over its three
+// tokens: the VALUE token takes P, the DUP token P+1 and the LEN token P+2
+// (when L > 2) -- so runs of length 2 and 3, the common ones, need no loop.
+// The LEN token's count is L-2; values P+3 .. P+L-1 of nonzero runs with
+// L >= 4 (listed by the loader) are added by the caller.
+template <int NT>
+struct Slots {
+  int val[NT];   // value contributed at pos (0 if none)
+  int pos[NT];   // output position of that value, global over the stream
+};
+
 // NT tokens per lane (a multiple of 4), chunks of 64*NT tokens.
 template <int NT>
 class StreamParser {
@@ -136,6 +148,83 @@ class StreamParser {
     for (int i = 0; i < NT; i++) ch.P[i] = p0 + pre[i];
   }
 
+  // Parses the chunk at `base` into per-token slots (see Slots).
+  template <bool PLAIN>
+  __device__ __forceinline__ void parse_slots(int base, Slots<NT>& s) {
+    int t[NT + 2];
+#pragma unroll
+    for (int q = 0; q < NV; q++) {
+      t[4 * q] = cur_[q].x;
+      t[4 * q + 1] = cur_[q].y;
+      t[4 * q + 2] = cur_[q].z;
+      t[4 * q + 3] = cur_[q].w;
+    }
+    const uint32_t nx0 = readlane((uint32_t)next_[0].x, 0);
+    const uint32_t nx1 = readlane((uint32_t)next_[0].y, 0);
+    t[NT] = (int)wave_shl1((uint32_t)t[0], nx0);
+    t[NT + 1] = (int)wave_shl1((uint32_t)t[1], nx1);
+    // the token before this lane's first (lane 0: the previous chunk's last)
+    const int tm1 = (int)wave_shr1((uint32_t)t[NT - 1], (uint32_t)prev_last_);
+    prev_last_ = (int)readlane((uint32_t)t[NT - 1], WAVE - 1);
+#pragma unroll
+    for (int q = 0; q < NV; q++) {
+      cur_[q] = next_[q];
+      next_[q] = *reinterpret_cast<const int4*>(tok_ + base + 2 * CH + NT * lane_ + 4 * q);
+    }
+    const int tid0 = base + NT * lane_;
+
+    uint32_t hzm = 0;
+    while (!PLAIN && next_hz_ < base + CH) {
+      const int o = next_hz_ - base;
+      if (lane_ == o / NT) hzm |= 1u << (o % NT);
+      hz_i_++;
+      next_hz_ = hz_i_ < nhz_ ? (int)hz_[hz_i_] : 0x7fffffff;
+    }
+
+    uint32_t f[NT];
+    bool valid[NT];
+#pragma unroll
+    for (int i = 0; i < NT; i++) {
+      if (PLAIN) {
+        valid[i] = true;
+        f[i] = t[i] == t[i + 1] ? FSM_EQ : FSM_NE;
+      } else {
+        valid[i] = tid0 + i >= lo_ && tid0 + i < len_;
+        const bool eq = !((hzm >> i) & 1u) && (tid0 + i + 1 < len_) && t[i] == t[i + 1];
+        f[i] = !valid[i] ? FSM_ID : (eq ? FSM_EQ : FSM_NE);
+      }
+    }
+    uint32_t F = f[0];
+#pragma unroll
+    for (int i = 1; i < NT; i++) F = fsm_compose(f[i], F);
+    const uint32_t Fin = wave_scan_fsm(F);
+    const uint32_t Fex = wave_shr1(Fin, FSM_ID);
+    uint32_t st = fsm_compose(Fex, state_);
+    state_ = fsm_compose(readlane(Fin, WAVE - 1), state_);
+
+    int lane_sum = 0, pre[NT];
+#pragma unroll
+    for (int i = 0; i < NT; i++) {
+      const int prev = i == 0 ? tm1 : t[i - 1];
+      const bool len_tok = st == FSM_CONST_LEN;
+      int cnt = len_tok ? t[i] - 2 : 1;
+      int val = len_tok ? (t[i] > 2 ? prev : 0) : t[i];
+      if (!PLAIN && !valid[i]) {
+        cnt = 0;
+        val = 0;
+      }
+      s.val[i] = val;
+      pre[i] = lane_sum;
+      lane_sum += cnt;
+      st = fsm_compose(f[i], st);
+    }
+    const int incl = wave_scan_add(lane_sum);
+    const int p0 = pos_ + incl - lane_sum;
+    pos_ += readlane(incl, WAVE - 1);
+#pragma unroll
+    for (int i = 0; i < NT; i++) s.pos[i] = p0 + pre[i];
+  }
+
   // Called after the last chunk: true if the stream produced nseg*512 values
   // and ended outside a run (always so for streams the loader accepted).
   __device__ bool complete(int nseg) const { return pos_ == nseg * BC && state_ == 0u; }
@@ -149,6 +238,7 @@ class StreamParser {
   int4 cur_[NV], next_[NV];
   int hz_i_ = 0, next_hz_;
   uint32_t state_ = 0u;  // constant function of the state entering the next chunk (0 = VALUE)
+  int prev_last_ = 0;    // last token of the previous chunk (parse_slots)
   int pos_ = 0;          // output position entering the next chunk
 };
 

@@ -29,6 +29,9 @@ constexpr int DPP_ROW_BCAST31 = 0x143;
 constexpr uint32_t FSM_ID = 0x03020100u;     // identity
 constexpr uint32_t FSM_NE = 0x03000200u;     // VALUE->VALUE, DUP->LEN, LEN->VALUE
 constexpr uint32_t FSM_EQ = 0x03000201u;     // VALUE->DUP,   DUP->LEN, LEN->VALUE
+// A state s is carried as the constant function s*0x01010101: composing any
+// function after it yields the constant function of the next state.
+constexpr uint32_t FSM_CONST_LEN = 0x02020202u;
 
 __device__ __forceinline__ int lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0)); }
 

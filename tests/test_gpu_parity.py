@@ -103,7 +103,29 @@ def hazard(tmp_path_factory, gpu):
     return _pair(H.basis_from_ints(dims, 6, hazard_zz, seed=12), tmp_path_factory, "hazard")
 
 
-CASES = ["smooth", "rough", "adversarial", "hazard"]
+def runs_zz(k, b, c):
+    """Nonzero runs of every length 1..7 at random offsets, so run tokens
+    (value, duplicate, length) straddle lane and chunk boundaries."""
+    rng = np.random.default_rng(7000 + 100 * k + 10 * b + c)
+    out, n = [], 0
+    while n < 512:
+        L = int(min(512 - n, rng.integers(1, 8)))
+        v = int(rng.choice([-3, -2, -1, 1, 2, 3, 0]))
+        if out and out[-1] == v:
+            v = v + 5  # keep runs maximal
+        out += [v] * L
+        n += L
+    return np.array(out, np.int32)
+
+
+@pytest.fixture(scope="module")
+def runs(tmp_path_factory, gpu):
+    dims = (16, 16, 8)
+    print(f"[case] nonzero runs of length 1..7 dims={dims} r=6", flush=True)
+    return _pair(H.basis_from_ints(dims, 6, runs_zz, seed=13), tmp_path_factory, "runs")
+
+
+CASES = ["smooth", "rough", "adversarial", "hazard", "runs"]
 
 
 def test_hazards_are_listed(hazard):
