@@ -30,6 +30,7 @@ hipError_t launch_step(const DevBasis& B, int mode, const float* v_in, const dou
                        float* field_out, const float* d_damp, const void* d_tab, int grid, hipStream_t s);
 hipError_t step_occupancy(int mode, int r, int nk, int* wg_per_cu);
 int step_max_cols();
+int step_partial_rows(int grid);
 size_t tables_bytes();
 void fill_tables(void* host_tab, const float* dct, const uint16_t* zig, const uint16_t* rev, const uint8_t* dz);
 hipError_t launch_decode(const DevBasis& B, int mode, const DecodeArgs& a, const float* d_damp, const void* d_tab,
@@ -363,6 +364,26 @@ int build_basis(const uint8_t* const bufs[3], const size_t sizes[3], cfs_basis**
   for (int b = 0; b < nB; b++) work[b] = (uint64_t)blk_len[b] + blk_len[(size_t)nB + b] + blk_len[(size_t)2 * nB + b];
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return work[a] > work[b]; });
 
+  // per-item descriptors of the fused step in schedule order (item i: block
+  // order[i / 3], component i % 3), 12 words each
+  std::vector<uint32_t> items((size_t)3 * nB * 12, 0);
+  for (size_t it = 0; it < (size_t)3 * nB; it++) {
+    const int b = order[it / 3], k = (int)(it % 3);
+    const size_t kb = (size_t)k * nB + b;
+    uint32_t* d = items.data() + it * 12;
+    d[0] = (uint32_t)blk_base[kb];
+    d[1] = (uint32_t)(blk_base[kb] >> 32);
+    d[2] = blk_len[kb];
+    d[3] = (uint32_t)kb;
+    d[4] = hz_start[kb];
+    d[5] = hz_start[kb + 1];
+    d[6] = lr_start[kb];
+    d[7] = lr_start[kb + 1];
+    std::memcpy(&d[8], &blk_bound[kb], 4);
+    d[9] = (uint32_t)b;
+    d[10] = (uint32_t)k;
+  }
+
   // damping table (1+d)^(k/4), FP64 -> FP32 (COMPRESSION_DATA.cpp:25-43),
   // scaled by 2^-FACTOR_SHIFT (and 1/s by 2^FACTOR_SHIFT) so that 22^32 fits FP32
   std::vector<float> damp((size_t)B->nk * NDIAG);
@@ -394,7 +415,7 @@ int build_basis(const uint8_t* const bufs[3], const size_t sizes[3], cfs_basis**
   int32_t* d_tok = nullptr;
   uint64_t* d_bb = nullptr;
   uint32_t *d_bl = nullptr, *d_lk = nullptr, *d_off = nullptr, *d_err = nullptr, *d_cnt = nullptr;
-  uint32_t *d_hzs = nullptr, *d_hz = nullptr, *d_lrs = nullptr, *d_lr = nullptr;
+  uint32_t *d_hzs = nullptr, *d_hz = nullptr, *d_lrs = nullptr, *d_lr = nullptr, *d_items = nullptr;
   float* d_vmax = nullptr;
   float* d_bound = nullptr;
   float *d_sinv = nullptr, *d_damp = nullptr;
@@ -423,10 +444,11 @@ int build_basis(const uint8_t* const bufs[3], const size_t sizes[3], cfs_basis**
   UP(d_hz, hz.data(), hz.size());
   UP(d_lrs, lr_start.data(), lr_start.size());
   UP(d_lr, lr.data(), lr.size());
+  UP(d_items, items.data(), items.size());
   const uint32_t zeros[4] = {0, 0, 0, 0};
   UP(d_err, zeros, 1);
   UP(d_cnt, zeros, 2);
-  UP(d_part, (const double*)nullptr, (size_t)maxgrid * r);
+  UP(d_part, (const double*)nullptr, (size_t)step_partial_rows(maxgrid) * r);
 #undef UP
   D.tokens = d_tok;
   D.blk_base = d_bb;
@@ -442,6 +464,7 @@ int build_basis(const uint8_t* const bufs[3], const size_t sizes[3], cfs_basis**
   D.hz = d_hz;
   D.lr_start = d_lrs;
   D.lr = d_lr;
+  D.items = reinterpret_cast<const uint4*>(d_items);
   D.err = d_err;
   D.counter = d_cnt;
   D.partials = d_part;

@@ -19,32 +19,40 @@ struct Tables {
   uint8_t dz[BC];       // dz[j] = u+v+w of the cell at zigzag position j
 };
 
-// Forward orthonormal 3D DCT of one zero-padded block of component k of the
-// full-grid interleaved field v (GetBlocksEigen + UnitaryBlockDCTEigen(1),
-// COMPRESSION.cpp:410-451, 536-555), written to vz in zigzag order.
-// `scr` is a 512-float LDS scratch.
-__device__ void forward_dct_block(const float* __restrict__ v, const DevBasis& B, int k, int b,
-                                  const float* __restrict__ C, const uint16_t* zig, float* scr, float* vz) {
+// Lane (y, z) gathers row (y, z) of the zero-padded peeled block b of
+// component k from the full-grid interleaved field v (GetBlocksEigen,
+// COMPRESSION.cpp:410-451).
+__device__ __forceinline__ void gather_block_rows(const float* __restrict__ v, const DevBasis& B, int k, int b,
+                                                  float a[BS]) {
   const int lane = lane_id();
   const int bx = b % B.nbx, by = (b / B.nbx) % B.nby, bz = b / (B.nbx * B.nby);
   const int GX = B.X + 2, GY = B.Y + 2;
+  const int y = lane & 7, z = lane >> 3;
+  const int gy = by * BS + y, gz = bz * BS + z;
+#pragma unroll
+  for (int x = 0; x < BS; x++) {
+    const int gx = bx * BS + x;
+    a[x] = (gx < B.X && gy < B.Y && gz < B.Z)
+               ? v[3 * ((size_t)(gx + 1) + (size_t)GX * ((gy + 1) + (size_t)GY * (gz + 1))) + k]
+               : 0.0f;
+  }
+}
+
+// Forward orthonormal 3D DCT (UnitaryBlockDCTEigen(1), COMPRESSION.cpp:536-555)
+// of a block whose rows were gathered by gather_block_rows, written to
+// vz[STRIDE * j] for zigzag position j.  `scr` is a 512-float LDS scratch.
+template <int STRIDE = 1>
+__device__ void forward_dct_rows(const float a0[BS], const float* __restrict__ C, const uint16_t* zig, float* scr,
+                                 float* vz) {
+  const int lane = lane_id();
   // pass along x: lane = (y, z)
   {
     const int y = lane & 7, z = lane >> 3;
-    const int gy = by * BS + y, gz = bz * BS + z;
-    float a[BS];
-#pragma unroll
-    for (int x = 0; x < BS; x++) {
-      const int gx = bx * BS + x;
-      a[x] = (gx < B.X && gy < B.Y && gz < B.Z)
-                 ? v[3 * ((size_t)(gx + 1) + (size_t)GX * ((gy + 1) + (size_t)GY * (gz + 1))) + k]
-                 : 0.0f;
-    }
 #pragma unroll
     for (int u = 0; u < BS; u++) {
       float s = 0.0f;
 #pragma unroll
-      for (int x = 0; x < BS; x++) s += C[u * BS + x] * a[x];
+      for (int x = 0; x < BS; x++) s += C[u * BS + x] * a0[x];
       scr[z * 64 + y * 8 + u] = s;
     }
   }
@@ -76,10 +84,19 @@ __device__ void forward_dct_block(const float* __restrict__ v, const DevBasis& B
       float s = 0.0f;
 #pragma unroll
       for (int z = 0; z < BS; z++) s += C[w * BS + z] * a[z];
-      vz[zig[u + 8 * vv + 64 * w]] = s;
+      vz[STRIDE * zig[u + 8 * vv + 64 * w]] = s;
     }
   }
   wave_lds_sync();
+}
+
+// Gather + forward DCT of block b of component k, written to vz in zigzag order.
+template <int STRIDE = 1>
+__device__ void forward_dct_block(const float* __restrict__ v, const DevBasis& B, int k, int b,
+                                  const float* __restrict__ C, const uint16_t* zig, float* scr, float* vz) {
+  float a[BS];
+  gather_block_rows(v, B, k, b, a);
+  forward_dct_rows<STRIDE>(a, C, zig, scr, vz);
 }
 
 // Inverse orthonormal 3D DCT of the zigzag-ordered fixed-point accumulator

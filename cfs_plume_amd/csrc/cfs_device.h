@@ -17,6 +17,7 @@
 //     2^-FACTOR_SHIFT and 1/s times 2^FACTOR_SHIFT: 22^32 = 9.4e42 exceeds the
 //     FP32 range, the scaled pair does not, and their product is unchanged.
 #pragma once
+#include <hip/hip_runtime.h>
 #include <stdint.h>
 
 namespace cfs {
@@ -51,6 +52,8 @@ struct DevBasis {
   const uint32_t* lr_start;   // [3*nB+1] first long run of block stream (k, b) in lr
   const uint32_t* lr;         // offsets of the length tokens of nonzero runs with L >= 4
   const int32_t* order;       // [nB] blocks in descending work order
+  const uint4* items;         // [3*nB][3] fused-step items in schedule order:
+                              //   {base lo, base hi, len, k*nB+b}, {hz0, hz1, lr0, lr1}, {bits(blk_bound), b, k, 0}
   uint32_t* err;              // device error bits
   uint32_t* counter;          // [2] spare device words (kept zero)
   double* partials;           // [grid][r] per-workgroup FP64 partial q

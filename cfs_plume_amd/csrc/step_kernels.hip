@@ -33,7 +33,7 @@ constexpr int MAX_COLS = 64 * QREG;
 constexpr int NO_END = 0x7fffffff;
 
 struct WaveLds {
-  int4* prm;     // [r+1] {damp row byte offset, bits(2^64/s), bits(fixed-point weight of q_c), end offset};
+  int4* prm;     // [r+1] {damp row byte offset, bits(fixed-point weight of q_c), bits(2^64/s), end offset};
                  // entry r is {0, 0, 0, NO_END} (tokens past the stream's end index it)
   double* qacc;  // [r] this wave's FP64 project accumulator
   float* vz;     // [512] DCT(v) block in zigzag order
@@ -42,10 +42,10 @@ struct WaveLds {
 };
 
 __host__ __device__ inline size_t step_wave_lds_bytes(int r) {
-  return ((size_t)(r + 1) * 16 + (size_t)r * 8 + 3 * BC * 4 + 15) / 16 * 16;
+  return (size_t)(r + 1) * 16 + 3 * (size_t)BC * 4 + ((size_t)r * 8 + 15) / 16 * 16;
 }
 __host__ __device__ inline size_t step_shared_lds_bytes(int nk) {
-  return ((size_t)nk * NDIAG * 4 + 15) / 16 * 16 + BC;
+  return ((size_t)nk * NDIAG * 4 + 15) / 16 * 16 + 3 * BC;  // damping rows, 4*(u+v+w), zigzag table
 }
 __host__ inline size_t step_lds_bytes(int r, int nk) {
   return step_shared_lds_bytes(nk) + WAVES_PER_WG * step_wave_lds_bytes(r);
@@ -62,17 +62,6 @@ __device__ __forceinline__ WaveLds carve_wave(char* base, int r) {
   return w;
 }
 
-// One value of the stream at output position pos: dequantise (int * damp *
-// (1/s), in the scaled units of the damping table) and feed both operators.
-template <bool PROJ, bool UNPROJ>
-__device__ __forceinline__ float step_value(const char* lds, const uint8_t* dz4, const WaveLds& w, const int4& pr,
-                                            int val, int j) {
-  const float dm = *reinterpret_cast<const float*>(lds + (pr.x + (int)dz4[j]));
-  const float vf = (float)val * dm;
-  if (UNPROJ) atomicAdd(&w.az[j], (int)(vf * __int_as_float(pr.z)));
-  return PROJ ? vf * w.vz[j] * __int_as_float(pr.y) : 0.0f;
-}
-
 // Unproject accumulates A[j] = sum_c q_c D_c[j] with integer LDS atomics
 // (gfx950's LDS float atomics are far slower).  |D_c[j]| <= vmax_c / s_c
 // (vmax_c from the loader), so |A[j]| <= max|q| * blk_bound with
@@ -80,9 +69,39 @@ __device__ __forceinline__ float step_value(const char* lds, const uint8_t* dz4,
 // partial sum of rounded terms stays below 2^31 and the rounding error is
 // below r units of 1/S per entry.
 //
-// Project keeps a running FP64 sum over the block stream (FP32 within a
-// chunk); the sum at the last token of segment c minus the sum at the last
-// token of segment c-1 is segment c's dot product, added to q_c.
+// Project keeps a running sum over the block stream of int * damp * DCT(v)
+// (FP32 within a chunk, FP64 across chunks).  The running sum at the last
+// token of segment c minus the sum at the last token of segment c-1, times
+// 1/s_c, is segment c's dot product.  The encoder scales every segment to the
+// same peak (2^(nBits-1)-1), so the unscaled terms of neighbouring segments
+// have comparable magnitudes.
+// Everything an item needs from global memory before its chunk loop; loaded
+// one item ahead so the latency overlaps the previous item's work.
+struct ItemPrefetch {
+  uint4 d0, d1, d2;     // the item's descriptor (wave-uniform)
+  uint32_t lk[QREG];    // seg_lk of columns lane + 64*m
+  float sv[QREG];       // seg_sinv of the same columns
+  float rows[BS];       // row (y, z) = lane of the block of v
+};
+
+template <bool PROJ>
+__device__ __forceinline__ void prefetch_item(const DevBasis& B, const float* __restrict__ v_in, int item,
+                                              ItemPrefetch& p) {
+  const uint4* d = B.items + (size_t)3 * item;
+  p.d0 = d[0];
+  p.d1 = d[1];
+  p.d2 = d[2];
+  const size_t i0 = (size_t)p.d0.w * B.r;
+  const int lane = lane_id();
+#pragma unroll
+  for (int m = 0; m < QREG; m++) {
+    const int c = lane + WAVE * m;
+    p.lk[m] = c < B.r ? B.seg_lk[i0 + c] : 0u;
+    p.sv[m] = c < B.r ? B.seg_sinv[i0 + c] : 0.0f;
+  }
+  if (PROJ) gather_block_rows(v_in, B, (int)p.d2.z, (int)p.d2.y, p.rows);
+}
+
 template <int MODE>
 __global__ void __launch_bounds__(WAVES_PER_WG * WAVE, 3)  // <= 168 VGPRs: three waves per SIMD
 step_kernel(DevBasis B, const float* __restrict__ v_in, const double* __restrict__ q_in,
@@ -93,16 +112,23 @@ step_kernel(DevBasis B, const float* __restrict__ v_in, const double* __restrict
 
   const int nk = B.nk;
   float* damp = reinterpret_cast<float*>(lds);
-  const size_t dbytes = ((size_t)nk * NDIAG * 4 + 15) / 16 * 16;
-  uint8_t* dz4 = reinterpret_cast<uint8_t*>(lds + dbytes);
-  const uint16_t* zig = g_tab->zig;  // used only by the per-block DCTs
+  uint8_t* dz4 = reinterpret_cast<uint8_t*>(lds + ((size_t)nk * NDIAG * 4 + 15) / 16 * 16);
+  uint16_t* zig = reinterpret_cast<uint16_t*>(dz4 + BC);  // used by the per-block DCTs
   const int wave = uniform((int)(threadIdx.x / WAVE));
   const int lane = lane_id();
   const WaveLds w = carve_wave(lds + step_shared_lds_bytes(nk) + (size_t)wave * step_wave_lds_bytes(B.r), B.r);
   const int r = B.r;
+  const int total = 3 * B.nB;
+  const int nwaves = gridDim.x * WAVES_PER_WG;
+  int item = blockIdx.x * WAVES_PER_WG + wave;
 
+  ItemPrefetch nx;
+  if (item < total) prefetch_item<PROJ>(B, v_in, item, nx);
   for (int i = threadIdx.x; i < nk * NDIAG; i += blockDim.x) damp[i] = g_damp[i];
-  for (int i = threadIdx.x; i < BC; i += blockDim.x) dz4[i] = (uint8_t)(4 * g_tab->dz[i]);
+  for (int i = threadIdx.x; i < BC; i += blockDim.x) {
+    dz4[i] = (uint8_t)(4 * g_tab->dz[i]);
+    zig[i] = g_tab->zig[i];
+  }
   for (int i = lane; i < BC; i += WAVE) w.az[i] = 0;
   for (int c = lane; c < r; c += WAVE) w.qacc[c] = 0.0;
   if (lane == 0) w.prm[r] = make_int4(0, 0, 0, NO_END);
@@ -120,31 +146,19 @@ step_kernel(DevBasis B, const float* __restrict__ v_in, const double* __restrict
   const float* C = g_tab->dct;
 
   bool complete = true;
-  const int total = 3 * B.nB;
-  const int nwaves = gridDim.x * WAVES_PER_WG;
   // Static round-robin over items sorted by decreasing work: the three
   // components of a block go to neighbouring waves and run together.
-  for (int item = blockIdx.x * WAVES_PER_WG + wave; item < total; item += nwaves) {
-    const int b = B.order[item / 3];
-    const int k = item % 3;
-    const size_t kb = (size_t)k * B.nB + b;
-    const size_t i0 = kb * r;
-    const int len = (int)B.blk_len[kb];
-    const uint64_t tbase = B.blk_base[kb];
-    const uint32_t h0 = B.hz_start[kb], h1 = B.hz_start[kb + 1];
-    const uint32_t l0 = B.lr_start[kb], l1 = B.lr_start[kb + 1];
-    // per-segment parameters: all loads first, one round trip
-    uint32_t lk[QREG];
-    float sv[QREG];
-#pragma unroll
-    for (int m = 0; m < QREG; m++) {
-      const int c = lane + WAVE * m;
-      lk[m] = c < r ? B.seg_lk[i0 + c] : 0u;
-      sv[m] = c < r ? B.seg_sinv[i0 + c] : 0.0f;
-    }
+  for (; item < total; item += nwaves) {
+    const ItemPrefetch it = nx;
+    const int b = (int)it.d2.y;
+    const int k = (int)it.d2.z;
+    const int len = (int)it.d0.z;
+    const uint64_t tbase = (uint64_t)it.d0.x | ((uint64_t)it.d0.y << 32);
+    const uint32_t h0 = it.d1.x, h1 = it.d1.y;
+    const uint32_t l0 = it.d1.z, l1 = it.d1.w;
     float inv_scale = 0.0f, S = 0.0f;
     if (UNPROJ) {
-      const float bound = B.blk_bound[kb] * qmax * 1.0001f;
+      const float bound = __uint_as_float(it.d2.x) * qmax * 1.0001f;
       S = bound > 0.0f ? 1073741824.0f / bound : 0.0f;
       inv_scale = S > 0.0f ? 1.0f / S : 0.0f;
     }
@@ -154,14 +168,14 @@ step_kernel(DevBasis B, const float* __restrict__ v_in, const double* __restrict
 #pragma unroll
     for (int m = 0; m < QREG; m++) {
       const int c = lane + WAVE * m;
-      const int L = (int)(lk[m] & 0xffffu);
+      const int L = (int)(it.lk[m] & 0xffffu);
       const int incl = wave_scan_add(L) + carry;
       carry = readlane(incl, WAVE - 1);
       if (c < r)
-        w.prm[c] = make_int4((int)(lk[m] >> 16) * (NDIAG * 4), __float_as_int(sv[m]), __float_as_int(qf[m] * sv[m] * S),
-                             incl);
+        w.prm[c] = make_int4((int)(it.lk[m] >> 16) * (NDIAG * 4), __float_as_int(qf[m] * it.sv[m] * S),
+                             __float_as_int(it.sv[m]), incl);
     }
-    if (PROJ) forward_dct_block(v_in, B, k, b, C, zig, w.pfx, w.vz);
+    if (PROJ) forward_dct_rows(it.rows, C, zig, w.pfx, w.vz);
     wave_lds_sync();
 
     StreamParser<NT> parser(B.tokens + tbase, 0, len, B.hz + h0, (int)(h1 - h0));
@@ -176,12 +190,38 @@ step_kernel(DevBasis B, const float* __restrict__ v_in, const double* __restrict
         parser.template parse_slots<true>(base, sl);
       else
         parser.template parse_slots<false>(base, sl);
+      // {bits(1/s), end} of the segments that may close in this chunk
+      int2 se = make_int2(0, NO_END);
+      if (PROJ) se = *reinterpret_cast<const int2*>(&w.prm[min(cur + lane, r)].z);
+      // Phases: all table reads first, then the arithmetic, then the atomics,
+      // so that the LDS round trips of the NT slots overlap.
       float acc = 0.0f, pre[NT];
+      {
+        int2 pr[NT];
+        int jj[NT], d4[NT];
+        float vzv[NT];
 #pragma unroll
-      for (int i = 0; i < NT; i++) {
-        const int4 pr = w.prm[sl.pos[i] >> 9];
-        acc += step_value<PROJ, UNPROJ>(lds, dz4, w, pr, sl.val[i], sl.pos[i] & (BC - 1));
-        pre[i] = acc;
+        for (int i = 0; i < NT; i++) {
+          jj[i] = sl.pos[i] & (BC - 1);
+          pr[i] = *reinterpret_cast<const int2*>(w.prm + (sl.pos[i] >> 9));
+          d4[i] = dz4[jj[i]];
+          vzv[i] = PROJ ? w.vz[jj[i]] : 0.0f;
+        }
+        float dm[NT];
+#pragma unroll
+        for (int i = 0; i < NT; i++) dm[i] = *reinterpret_cast<const float*>(lds + (pr[i].x + d4[i]));
+        int qv[NT];
+#pragma unroll
+        for (int i = 0; i < NT; i++) {
+          const float vf = (float)sl.val[i] * dm[i];
+          if (PROJ) acc += vf * vzv[i];
+          pre[i] = acc;
+          qv[i] = (int)(vf * __int_as_float(pr[i].y));
+        }
+        if (UNPROJ) {
+#pragma unroll
+          for (int i = 0; i < NT; i++) atomicAdd(&w.az[jj[i]], qv[i]);
+        }
       }
       // values P+3 .. P+L-1 of nonzero runs with L >= 4 (listed at load)
       while (next_lr < base + CH) {
@@ -194,10 +234,14 @@ step_kernel(DevBasis B, const float* __restrict__ v_in, const double* __restrict
         pv = readlane(pv, ol);                               // position of the value at P+2
         const int Lv = B.tokens[tbase + next_lr];            // run length
         const int vv = B.tokens[tbase + next_lr - 1];        // run value (the DUP token)
-        const int4 pr = w.prm[pv >> 9];
+        const int2 pr = *reinterpret_cast<const int2*>(w.prm + (pv >> 9));
         float ex = 0.0f;
-        for (int m = lane; m < Lv - 3; m += WAVE)
-          ex += step_value<PROJ, UNPROJ>(lds, dz4, w, pr, vv, (pv + 1 + m) & (BC - 1));
+        for (int m = lane; m < Lv - 3; m += WAVE) {
+          const int j = (pv + 1 + m) & (BC - 1);
+          const float vf = (float)vv * *reinterpret_cast<const float*>(lds + (pr.x + (int)dz4[j]));
+          if (PROJ) ex += vf * w.vz[j];
+          if (UNPROJ) atomicAdd(&w.az[j], (int)(vf * __int_as_float(pr.y)));
+        }
         if (PROJ) {
           ex = wave_sum(ex);
           if (lane == ol) {
@@ -221,24 +265,31 @@ step_kernel(DevBasis B, const float* __restrict__ v_in, const double* __restrict
         // segments cur, cur+1, ... whose last token lies in this chunk
         for (;;) {
           const int c = cur + lane;
-          const int e = w.prm[min(c, r)].w;
-          const bool closes = e <= base + CH;
-          const int pi = closes ? e - 1 - base : 0;
-          const double endv = closes ? run + (double)w.pfx[pi] : 0.0;
-          const double prevv = __shfl_up(endv, 1);
-          const double d = endv - (lane == 0 ? last_end : prevv);
-          if (closes) w.qacc[c] += d;
-          const uint64_t mask = __ballot(closes);
-          const int n = __popcll(mask);
-          if (n > 0) last_end = __shfl(endv, n - 1);
+          const bool closes = se.y <= base + CH;
+          const int pi = closes ? se.y - 1 - base : 0;
+          const float pv = w.pfx[pi];
+          const double q0 = closes ? w.qacc[c] : 0.0;
+          const double endv = run + (double)pv;
+          // running sum at the previous segment's end: lane-1's, or the last one closed
+          const int2 eb = make_int2(__double2loint(endv), __double2hiint(endv));
+          const double prevv = __hiloint2double((int)wave_shr1((uint32_t)eb.y, (uint32_t)__double2hiint(last_end)),
+                                                (int)wave_shr1((uint32_t)eb.x, (uint32_t)__double2loint(last_end)));
+          if (closes) w.qacc[c] = q0 + (endv - prevv) * (double)__int_as_float(se.x);
+          const int n = __popcll(__ballot(closes));
+          if (n > 0)
+            last_end = __hiloint2double(readlane(eb.y, n - 1), readlane(eb.x, n - 1));
           cur += n;
           if (n < WAVE) break;
+          se = *reinterpret_cast<const int2*>(&w.prm[min(cur + lane, r)].z);
         }
         run += (double)readlanef(incl, WAVE - 1);
         wave_lds_sync();
       }
     }
     complete = complete && parser.complete(r);
+    // the next item's descriptor, segment parameters and block of v, in
+    // flight during this item's inverse DCT
+    if (item + nwaves < total) prefetch_item<PROJ>(B, v_in, item + nwaves, nx);
     wave_lds_sync();
 
     if (UNPROJ) inverse_dct_block(field_out, B, k, b, C, zig, w.az, inv_scale, w.pfx);
@@ -283,6 +334,8 @@ namespace cfs {
 
 // largest r the step kernel supports (q accumulators live in registers)
 int step_max_cols() { return MAX_COLS; }
+// rows of the FP64 partial-q buffer a launch of `grid` workgroups needs
+int step_partial_rows(int grid) { return grid; }
 
 hipError_t launch_step(const DevBasis& B, int mode, const float* v_in, const double* q_in, double* q_out,
                        float* field_out, const float* d_damp, const void* d_tab, int grid, hipStream_t s) {

