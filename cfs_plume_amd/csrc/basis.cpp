@@ -237,6 +237,9 @@ int build_basis(const uint8_t* const bufs[3], const size_t sizes[3], cfs_basis**
         f[k].nB != f[0].nB)
       return fail(CFS_ERR_FORMAT, "component headers disagree");
 
+  if (3.0 * (f[0].X + 2.0) * (f[0].Y + 2.0) * (f[0].Z + 2.0) >= 2147483647.0)
+    return fail(CFS_ERR_FORMAT, "grid too large: a full-grid field must hold fewer than 2^31 floats");
+
   std::unique_ptr<cfs_basis> B(new cfs_basis);
   B->nBits = f[0].nBits;
   B->X = f[0].X; B->Y = f[0].Y; B->Z = f[0].Z;

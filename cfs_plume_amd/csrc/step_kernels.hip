@@ -143,8 +143,6 @@ step_kernel(DevBasis B, const float* __restrict__ v_in, const double* __restrict
   }
   if (UNPROJ) qmax = wave_max(qmax);
   __syncthreads();
-  const float* C = g_tab->dct;
-
   bool complete = true;
   // Static round-robin over items sorted by decreasing work: the three
   // components of a block go to neighbouring waves and run together.
@@ -175,7 +173,7 @@ step_kernel(DevBasis B, const float* __restrict__ v_in, const double* __restrict
         w.prm[c] = make_int4((int)(it.lk[m] >> 16) * (NDIAG * 4), __float_as_int(qf[m] * it.sv[m] * S),
                              __float_as_int(it.sv[m]), incl);
     }
-    if (PROJ) forward_dct_rows(it.rows, C, zig, w.pfx, w.vz);
+    if (PROJ) forward_dct_rows(it.rows, zig, w.pfx, w.vz);
     wave_lds_sync();
 
     StreamParser<NT> parser(B.tokens + tbase, 0, len, B.hz + h0, (int)(h1 - h0));
@@ -292,7 +290,7 @@ step_kernel(DevBasis B, const float* __restrict__ v_in, const double* __restrict
     if (item + nwaves < total) prefetch_item<PROJ>(B, v_in, item + nwaves, nx);
     wave_lds_sync();
 
-    if (UNPROJ) inverse_dct_block(field_out, B, k, b, C, zig, w.az, inv_scale, w.pfx);
+    if (UNPROJ) inverse_dct_block(field_out, B, k, b, zig, w.az, inv_scale, w.pfx);
     wave_lds_sync();
   }
   if (!complete && lane == 0) atomicOr(B.err, (uint32_t)ERR_INCOMPLETE);
