@@ -33,6 +33,7 @@ int step_max_cols();
 int step_partial_rows(int grid);
 size_t tables_bytes();
 void fill_tables(void* host_tab, const float* dct, const uint16_t* zig, const uint16_t* rev, const uint8_t* dz);
+void fill_mfma_frags(void* host_tab);
 hipError_t launch_decode(const DevBasis& B, int mode, const DecodeArgs& a, const float* d_damp, const void* d_tab,
                          hipStream_t s);
 }  // namespace cfs
@@ -359,6 +360,19 @@ int build_basis(const uint8_t* const bufs[3], const size_t sizes[3], cfs_basis**
   }
   lr_start[(size_t)3 * nB] = (uint32_t)lr.size();
   B->nlong = (int64_t)lr.size();
+  // per segment: index of its first long run (segments are block-major, as lr)
+  std::vector<uint32_t> seg_lr(nseg + 1);
+  parallel_for((size_t)3 * nB, [&](size_t a, size_t bnd) {
+    for (size_t kb = a; kb < bnd; kb++) {
+      uint32_t li = lr_start[kb];
+      for (int c = 0; c < r; c++) {
+        const size_t si = kb * r + c;
+        while (li < lr_start[kb + 1] && lr[li] < seg_off[si]) li++;
+        seg_lr[si] = li;
+      }
+    }
+  });
+  seg_lr[nseg] = (uint32_t)lr.size();
 
   // work order: blocks with the longest streams first
   std::vector<int32_t> order(nB);
@@ -396,6 +410,7 @@ int build_basis(const uint8_t* const bufs[3], const size_t sizes[3], cfs_basis**
   std::vector<uint8_t> tab(tables_bytes());
   const HostTables& T = host_tables();
   fill_tables(tab.data(), T.dct, T.zig, T.rev, T.dz);
+  fill_mfma_frags(tab.data());
 
   // persistent grid sizes
   hipError_t e = hipGetDevice(&B->device);
@@ -418,7 +433,8 @@ int build_basis(const uint8_t* const bufs[3], const size_t sizes[3], cfs_basis**
   int32_t* d_tok = nullptr;
   uint64_t* d_bb = nullptr;
   uint32_t *d_bl = nullptr, *d_lk = nullptr, *d_off = nullptr, *d_err = nullptr, *d_cnt = nullptr;
-  uint32_t *d_hzs = nullptr, *d_hz = nullptr, *d_lrs = nullptr, *d_lr = nullptr, *d_items = nullptr;
+  uint32_t *d_hzs = nullptr, *d_hz = nullptr, *d_lrs = nullptr, *d_lr = nullptr, *d_items = nullptr,
+           *d_seglr = nullptr;
   float* d_vmax = nullptr;
   float* d_bound = nullptr;
   float *d_sinv = nullptr, *d_damp = nullptr;
@@ -448,6 +464,7 @@ int build_basis(const uint8_t* const bufs[3], const size_t sizes[3], cfs_basis**
   UP(d_lrs, lr_start.data(), lr_start.size());
   UP(d_lr, lr.data(), lr.size());
   UP(d_items, items.data(), items.size());
+  UP(d_seglr, seg_lr.data(), seg_lr.size());
   const uint32_t zeros[4] = {0, 0, 0, 0};
   UP(d_err, zeros, 1);
   UP(d_cnt, zeros, 2);
@@ -468,6 +485,7 @@ int build_basis(const uint8_t* const bufs[3], const size_t sizes[3], cfs_basis**
   D.lr_start = d_lrs;
   D.lr = d_lr;
   D.items = reinterpret_cast<const uint4*>(d_items);
+  D.seg_lr = d_seglr;
   D.err = d_err;
   D.counter = d_cnt;
   D.partials = d_part;

@@ -11,12 +11,24 @@
 
 namespace cfs {
 
+// Per-lane constant operands of the MFMA inverse DCT (decode_kernels.hip),
+// as BF16 bit patterns split hi + lo (value = hi + lo to ~2^-17):
+//   s1b[h][lane][i]: B operand of the x pass, blockdiag(C, C) (16 x 16),
+//                    element (k = 4*(lane/16) + i, n = lane % 16);
+//   s2a[h][t][ks][lane][i]: A operand of the joint (y, z) pass, rows
+//                    (z, y) = 16 t + lane % 16, K index 32 ks + 8*(lane/16) + i.
+struct MfmaFrags {
+  uint16_t s1b[2][64][4];
+  uint16_t s2a[2][4][2][64][8];
+};
+
 // Small read-only tables, shared by every basis on the device.
 struct Tables {
   float dct[64];        // C[k][n] = a_k cos(pi (2n+1) k / 16), row k
   uint16_t zig[BC];     // zig[x + 8y + 64z] = zigzag position (COMPRESSION_DATA.cpp:90-120)
   uint16_t rev[BC];     // rev[j] = x + 8y + 64z of zigzag position j
   uint8_t dz[BC];       // dz[j] = u+v+w of the cell at zigzag position j
+  MfmaFrags mf;
 };
 
 // Orthonormal 8-point DCT constants c_m = a cos(m pi / 16), rounded to FP32

@@ -51,6 +51,7 @@ struct DevBasis {
   const uint32_t* hz;         // hazard offsets (segment-final singleton equal to the next token)
   const uint32_t* lr_start;   // [3*nB+1] first long run of block stream (k, b) in lr
   const uint32_t* lr;         // offsets of the length tokens of nonzero runs with L >= 4
+  const uint32_t* seg_lr;     // [3*nB*r + 1] index in lr of the first long run of each segment (block-major)
   const int32_t* order;       // [nB] blocks in descending work order
   const uint4* items;         // [3*nB][3] fused-step items in schedule order:
                               //   {base lo, base hi, len, k*nB+b}, {hz0, hz1, lr0, lr1}, {bits(blk_bound), b, k, 0}

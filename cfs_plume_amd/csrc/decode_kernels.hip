@@ -11,6 +11,10 @@
 //               the 3 x r submatrix          GetSubmatrixNoSVDSparse (:1949-2158)
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
 #include "block_dct.h"
 #include "cfs_device.h"
 #include "stream_decode.h"
@@ -138,8 +142,295 @@ decode_kernel(DevBasis B, DecodeArgs a, const float* __restrict__ g_damp, const 
   }
 }
 
+// ---------------------------------------------------------------------------
+// Spatial decode of whole columns (DecodeVectorField, COMPRESSION.cpp:2640-2670)
+// with the inverse DCT on MFMA, BF16 hi + lo split products, FP32 accumulation.
+//
+// Work item: (column group of SP_G columns, block b), one workgroup of three
+// waves, wave k decoding component k.  A group's segments are contiguous
+// inside block stream (k, b), so each wave parses them as one stream and
+// scatters the dequantised coefficients into LDS (natural u + 8v + 64w
+// order, one 512-float buffer per column).  Then, per pair of columns, each
+// wave transforms its component:
+//   x pass   T = X C           rows (w, v), K = (block, u) = 16: 4 tiles of
+//                              v_mfma_f32_16x16x16_bf16 against blockdiag(C, C)
+//   (y, z)   Out = K T         K[(z,y),(w,v)] = C[w][z] C[v][y] (64 x 64): 4 tiles x
+//                              2 K-steps of v_mfma_f32_16x16x32_bf16.
+// The x pass's accumulator layout (lane: rows 4g..4g+3 of column n) is used
+// as the (y, z) pass's B operand as it stands: K index 32 ks + 8 g + i maps
+// to row (w, v) = 32 ks + 4 g + i (i < 4) or 32 ks + 16 + 4 g + i - 4, and the
+// host orders the columns of K the same way.  Each product of a hi + lo pair
+// keeps hi*hi + hi*lo + lo*hi.  The three components meet in an LDS tile of
+// interleaved cells, written to the column as contiguous 96-byte rows.
+// ---------------------------------------------------------------------------
+typedef short bf16x4_t __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+
+constexpr int SP_G = 4;    // columns per work item
+constexpr int SP_NT = 4;   // tokens per lane per chunk of the parser
+
+__device__ __forceinline__ short bf16_bits(__bf16 h) {
+  short v;
+  __builtin_memcpy(&v, &h, 2);
+  return v;
+}
+// x = hi + lo, both BF16 (round to nearest even)
+__device__ __forceinline__ void split_bf16(float x, short& hi, short& lo) {
+  const __bf16 h = (__bf16)x;
+  hi = bf16_bits(h);
+  lo = bf16_bits((__bf16)(x - (float)h));
+}
+__device__ __forceinline__ bf16x8_t as_bf16x8(const int4& v) {
+  bf16x8_t r;
+  __builtin_memcpy(&r, &v, 16);
+  return r;
+}
+
+// LDS: damping rows | jinfo[512] | per component: X[SP_G][512] + prm[SP_G+1] | cell tile [2][512][3]
+__host__ __device__ inline size_t spatial_shared_lds_bytes(int nk) { return ((size_t)nk * NDIAG * 4 + 15) / 16 * 16 + BC * 4; }
+__host__ __device__ inline size_t spatial_comp_lds_bytes() { return (size_t)SP_G * BC * 4 + (SP_G + 2) * 8; }
+__host__ inline size_t spatial_lds_bytes(int nk) {
+  return spatial_shared_lds_bytes(nk) + 3 * spatial_comp_lds_bytes() + (size_t)2 * BC * 3 * 4;
+}
+
+__global__ void __launch_bounds__(3 * WAVE)
+spatial_kernel(DevBasis B, DecodeArgs a, const float* __restrict__ g_damp, const Tables* __restrict__ g_tab) {
+  extern __shared__ __align__(16) char lds[];
+  const int nk = B.nk;
+  float* damp = reinterpret_cast<float*>(lds);
+  // per zigzag position j: natural index rev[j] | 4*(u+v+w) << 16
+  uint32_t* jinfo = reinterpret_cast<uint32_t*>(lds + ((size_t)nk * NDIAG * 4 + 15) / 16 * 16);
+  const int k = uniform((int)(threadIdx.x / WAVE));  // this wave's component
+  const int lane = lane_id();
+  char* cbase = lds + spatial_shared_lds_bytes(nk) + (size_t)k * spatial_comp_lds_bytes();
+  float* X = reinterpret_cast<float*>(cbase);                           // [SP_G][512]
+  int2* prm = reinterpret_cast<int2*>(cbase + (size_t)SP_G * BC * 4);   // [SP_G + 1] {row byte offset, bits(1/s)}
+  float* tile = reinterpret_cast<float*>(lds + spatial_shared_lds_bytes(nk) + 3 * spatial_comp_lds_bytes());
+
+  for (int i = threadIdx.x; i < nk * NDIAG; i += blockDim.x) damp[i] = g_damp[i];
+  for (int i = threadIdx.x; i < BC; i += blockDim.x)
+    jinfo[i] = (uint32_t)g_tab->rev[i] | ((uint32_t)(4 * g_tab->dz[i]) << 16);
+
+  const int r = B.r;
+  const int ngroups = (a.ncol + SP_G - 1) / SP_G;
+  const int grp = (int)(blockIdx.x / (unsigned)B.nB);
+  const int b = (int)(blockIdx.x % (unsigned)B.nB);
+  const int c0 = a.col0 + grp * SP_G;
+  const int G = min(SP_G, a.col0 + a.ncol - c0);
+  const size_t kb = (size_t)k * B.nB + b;
+  const size_t si0 = kb * r + c0;
+  (void)ngroups;
+
+  // column c0 + lane's segment: parameters and its token range in the block stream
+  uint32_t lk = 0, off = 0;
+  float sv = 0.0f;
+  if (lane < G) {
+    lk = B.seg_lk[si0 + lane];
+    off = B.seg_off[si0 + lane];
+    sv = B.seg_sinv[si0 + lane];
+  }
+  const int lo = readlane((int)off, 0);
+  const int hi = readlane((int)off, G - 1) + (int)(readlane(lk, G - 1) & 0xffffu);
+  if (lane <= G) prm[lane] = lane < G ? make_int2((int)(lk >> 16) * (NDIAG * 4), __float_as_int(sv)) : make_int2(0, 0);
+  {
+    float4* x4 = reinterpret_cast<float4*>(X);
+    for (int i = lane; i < SP_G * BC / 4; i += WAVE) x4[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  }
+  __syncthreads();  // tables and this wave's buffers
+
+  // parse the group's tokens as one stream; tokens outside [lo, hi) are ignored
+  const int lo_al = lo & ~3;
+  const int32_t* tok = B.tokens + B.blk_base[kb] + lo_al;
+  const uint32_t h0 = B.hz_start[kb], h1 = B.hz_start[kb + 1];
+  StreamParser<SP_NT> parser(tok, lo - lo_al, hi - lo_al, B.hz + h0, (int)(h1 - h0), lo_al);
+  // long runs of columns c0 .. c0+G-1: [seg_lr[si0], seg_lr[si0 + G]) (seg_lr has one extra entry)
+  uint32_t li = B.seg_lr[si0];
+  const uint32_t l1 = B.seg_lr[si0 + G];
+  int next_lr = li < l1 ? (int)B.lr[li] - lo_al : 0x7fffffff;
+  for (int base = 0; base < hi - lo_al; base += 64 * SP_NT) {
+    Slots<SP_NT> sl;
+    if (parser.plain(base))
+      parser.template parse_slots<true>(base, sl);
+    else
+      parser.template parse_slots<false>(base, sl);
+#pragma unroll
+    for (int i = 0; i < SP_NT; i++) {
+      if (sl.val[i] != 0) {
+        const int2 pr = prm[sl.pos[i] >> 9];
+        const uint32_t ji = jinfo[sl.pos[i] & (BC - 1)];
+        const float dm = *reinterpret_cast<const float*>(lds + (pr.x + (int)(ji >> 16)));
+        X[(sl.pos[i] >> 9) * BC + (int)(ji & 0xffffu)] = (float)sl.val[i] * (dm * __int_as_float(pr.y));
+      }
+    }
+    // values P+3 .. P+L-1 of nonzero runs with L >= 4
+    while (next_lr < min(base + 64 * SP_NT, hi - lo_al)) {
+      const int o = next_lr - base;
+      const int ol = o / SP_NT, os = o % SP_NT;
+      int pv = 0;
+#pragma unroll
+      for (int i = 0; i < SP_NT; i++)
+        if (i == os) pv = sl.pos[i];
+      pv = readlane(pv, ol);
+      const int Lv = tok[next_lr];
+      const int vv = tok[next_lr - 1];
+      const int2 pr = prm[pv >> 9];
+      for (int m = lane; m < Lv - 3; m += WAVE) {
+        const int p = pv + 1 + m;
+        const uint32_t ji = jinfo[p & (BC - 1)];
+        const float dm = *reinterpret_cast<const float*>(lds + (pr.x + (int)(ji >> 16)));
+        X[(p >> 9) * BC + (int)(ji & 0xffffu)] = (float)vv * (dm * __int_as_float(pr.y));
+      }
+      li++;
+      next_lr = li < l1 ? (int)B.lr[li] - lo_al : 0x7fffffff;
+    }
+  }
+  if (!parser.complete(G) && lane == 0) atomicOr(B.err, (uint32_t)ERR_INCOMPLETE);
+  wave_lds_sync();
+
+  // constant MFMA operands of this lane
+  bf16x4_t b1h, b1l;
+  bf16x8_t a2h[4][2], a2l[4][2];
+  {
+    const MfmaFrags& mf = g_tab->mf;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      b1h[i] = (short)mf.s1b[0][lane][i];
+      b1l[i] = (short)mf.s1b[1][lane][i];
+    }
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+#pragma unroll
+      for (int ks = 0; ks < 2; ks++) {
+        a2h[t][ks] = as_bf16x8(*reinterpret_cast<const int4*>(mf.s2a[0][t][ks][lane]));
+        a2l[t][ks] = as_bf16x8(*reinterpret_cast<const int4*>(mf.s2a[1][t][ks][lane]));
+      }
+  }
+
+  const int g = lane >> 4, n = lane & 15;
+  const int bx = b % B.nbx, by = (b / B.nbx) % B.nby, bz = b / (B.nbx * B.nby);
+  const size_t col_stride = (size_t)3 * B.X * B.Y * B.Z;
+  for (int s0 = 0; s0 < G; s0 += 2) {
+    // x pass: lane (m = n, g) reads row 16t + n of block s0 + (g >> 1), u = 4 (g & 1) .. +3
+    const float* xs = X + (s0 + (g >> 1)) * BC + 4 * (g & 1);
+    f32x4_t d1[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+      const float4 v = *reinterpret_cast<const float4*>(xs + (16 * t + n) * BS);
+      short h0, h1, h2, h3, l0, l1, l2, l3;
+      split_bf16(v.x, h0, l0);
+      split_bf16(v.y, h1, l1);
+      split_bf16(v.z, h2, l2);
+      split_bf16(v.w, h3, l3);
+      const bf16x4_t ah = {h0, h1, h2, h3}, al = {l0, l1, l2, l3};
+      f32x4_t c = {0.0f, 0.0f, 0.0f, 0.0f};
+      c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, b1h, c, 0, 0, 0);
+      c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, b1l, c, 0, 0, 0);
+      c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, b1h, c, 0, 0, 0);
+      d1[t] = c;
+    }
+    // (y, z) pass: B operand straight from the x pass's accumulators
+    bf16x8_t b2h[2], b2l[2];
+#pragma unroll
+    for (int ks = 0; ks < 2; ks++)
+#pragma unroll
+      for (int i = 0; i < 8; i++) {
+        short h, l;
+        split_bf16(i < 4 ? d1[2 * ks][i] : d1[2 * ks + 1][i - 4], h, l);
+        __bf16 hb, lb;
+        __builtin_memcpy(&hb, &h, 2);
+        __builtin_memcpy(&lb, &l, 2);
+        b2h[ks][i] = hb;
+        b2l[ks][i] = lb;
+      }
+    // lane column n = (block s0 + n / 8, x = n % 8), rows (z, y) = 16 t + 4 g + i;
+    // tile[(block, cell x + 8y + 64z)][k]
+    float* tp = tile + ((size_t)(n >> 3) * BC + (n & 7)) * 3 + k;
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+      f32x4_t c = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+      for (int ks = 0; ks < 2; ks++) {
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2h[t][ks], b2h[ks], c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2h[t][ks], b2l[ks], c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2l[t][ks], b2h[ks], c, 0, 0, 0);
+      }
+#pragma unroll
+      for (int i = 0; i < 4; i++) tp[(16 * t + 4 * g + i) * BS * 3] = c[i];
+    }
+    __syncthreads();  // the three components of the pair's cells are in the tile
+    // rows of 8 cells x 3 components = 24 contiguous floats per (block, z, y)
+    {
+      const int f24 = threadIdx.x % 24;
+      const int x = f24 / 3;
+      const int gx = bx * BS + x;
+      for (int row = threadIdx.x / 24; row < 2 * 64; row += (3 * WAVE) / 24) {
+        const int sb = row >> 6, q = row & 63;
+        const int gy = by * BS + (q & 7), gz = bz * BS + (q >> 3);
+        if (s0 + sb < G && gx < B.X && gy < B.Y && gz < B.Z) {
+          float* o = reinterpret_cast<float*>(a.out) + (size_t)(c0 + s0 + sb - a.col0) * col_stride;
+          o[3 * (gx + B.X * (gy + B.Y * gz)) + f24 % 3] = tile[(sb * BC + q * BS) * 3 + f24];
+        }
+      }
+    }
+    __syncthreads();  // the tile is free for the next pair
+  }
+}
+
+// Host: the MFMA operands of the inverse DCT in lane order (FP64 closed form
+// split into BF16 hi + lo, round to nearest even).
+static uint16_t host_bf16_bits(float x) {
+  uint32_t u;
+  std::memcpy(&u, &x, 4);
+  return (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+static float host_bf16_value(uint16_t h) {
+  const uint32_t u = (uint32_t)h << 16;
+  float f;
+  std::memcpy(&f, &u, 4);
+  return f;
+}
+static void host_split(double x, uint16_t* hi, uint16_t* lo) {
+  *hi = host_bf16_bits((float)x);
+  *lo = host_bf16_bits((float)(x - (double)host_bf16_value(*hi)));
+}
+
+void fill_mfma_frags(void* host_tab) {
+  MfmaFrags& mf = reinterpret_cast<Tables*>(host_tab)->mf;
+  double C[BS][BS];
+  for (int kk = 0; kk < BS; kk++)
+    for (int nn = 0; nn < BS; nn++)
+      C[kk][nn] = (kk == 0 ? std::sqrt(0.125) : 0.5) * std::cos(M_PI * (double)((2 * nn + 1) * kk) / 16.0);
+  for (int l = 0; l < WAVE; l++) {
+    const int g = l / 16, nn = l % 16;
+    for (int i = 0; i < 4; i++) {
+      const int kk = 4 * g + i;  // blockdiag(C, C)[kk][nn]
+      const double v = ((kk < 8) == (nn < 8)) ? C[kk % 8][nn % 8] : 0.0;
+      host_split(v, &mf.s1b[0][l][i], &mf.s1b[1][l][i]);
+    }
+    for (int t = 0; t < 4; t++)
+      for (int ks = 0; ks < 2; ks++)
+        for (int i = 0; i < 8; i++) {
+          const int q = 16 * t + nn;                                         // output row (z, y)
+          const int rw = i < 4 ? 32 * ks + 4 * g + i : 32 * ks + 16 + 4 * g + (i - 4);  // input row (w, v)
+          const double v = C[rw >> 3][q >> 3] * C[rw & 7][q & 7];             // C[w][z] C[v][y]
+          host_split(v, &mf.s2a[0][t][ks][l][i], &mf.s2a[1][t][ks][l][i]);
+        }
+  }
+}
+
 hipError_t launch_decode(const DevBasis& B, int mode, const DecodeArgs& a, const float* d_damp, const void* d_tab,
                          hipStream_t s) {
+  if (mode == DEC_SPATIAL) {
+    const long items = (long)((a.ncol + SP_G - 1) / SP_G) * B.nB;
+    if (items <= 0 || a.k0 != 0 || a.nk3 != 3 || a.b0 != 0 || a.nblk != B.nB) return hipErrorInvalidValue;
+    const size_t lds = spatial_lds_bytes(B.nk);
+    const long grid = (long)((a.ncol + SP_G - 1) / SP_G) * B.nB;  // one workgroup per (column group, block)
+    if (grid > 0x7fffffffL) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(spatial_kernel, dim3((unsigned)grid), dim3(3 * WAVE), lds, s, B, a, d_damp,
+                       reinterpret_cast<const Tables*>(d_tab));
+    return hipGetLastError();
+  }
   const long total = (long)a.ncol * a.nk3 * a.nblk;
   if (total <= 0) return hipSuccess;
   const dim3 grid((unsigned)((total + DEC_WAVES - 1) / DEC_WAVES));

@@ -54,15 +54,18 @@ class StreamParser {
   // tok must be 16-byte aligned; tokens outside [lo, hi) are ignored.
   // hz/nhz: sorted offsets (from tok) of segment-final singletons that equal
   // the next token; the parser treats them as singletons.
-  __device__ StreamParser(const int32_t* __restrict__ tok, int lo, int hi, const uint32_t* __restrict__ hz, int nhz)
-      : tok_(tok), lo_(lo), len_(hi), hz_(hz), nhz_(nhz) {
+  // hz_bias is subtracted from every hazard offset (lists kept relative to
+  // an earlier origin than tok).
+  __device__ StreamParser(const int32_t* __restrict__ tok, int lo, int hi, const uint32_t* __restrict__ hz, int nhz,
+                          int hz_bias = 0)
+      : tok_(tok), lo_(lo), len_(hi), hz_(hz), nhz_(nhz), hz_bias_(hz_bias) {
     lane_ = lane_id();
 #pragma unroll
     for (int q = 0; q < NV; q++) {
       cur_[q] = *reinterpret_cast<const int4*>(tok_ + NT * lane_ + 4 * q);
       next_[q] = *reinterpret_cast<const int4*>(tok_ + CH + NT * lane_ + 4 * q);
     }
-    next_hz_ = nhz_ > 0 ? (int)hz_[0] : 0x7fffffff;
+    next_hz_ = nhz_ > 0 ? (int)hz_[0] - hz_bias_ : 0x7fffffff;
   }
 
   // True if the chunk at `base` lies inside [lo, hi - 1) and holds no hazard,
@@ -100,9 +103,9 @@ class StreamParser {
     uint32_t hzm = 0;
     while (!PLAIN && next_hz_ < base + CH) {
       const int o = next_hz_ - base;
-      if (lane_ == o / NT) hzm |= 1u << (o % NT);
+      if (o >= 0 && lane_ == o / NT) hzm |= 1u << (o % NT);  // o < 0: before this range
       hz_i_++;
-      next_hz_ = hz_i_ < nhz_ ? (int)hz_[hz_i_] : 0x7fffffff;
+      next_hz_ = hz_i_ < nhz_ ? (int)hz_[hz_i_] - hz_bias_ : 0x7fffffff;
     }
 
     // transition functions and the lane's composed function
@@ -176,9 +179,9 @@ class StreamParser {
     uint32_t hzm = 0;
     while (!PLAIN && next_hz_ < base + CH) {
       const int o = next_hz_ - base;
-      if (lane_ == o / NT) hzm |= 1u << (o % NT);
+      if (o >= 0 && lane_ == o / NT) hzm |= 1u << (o % NT);  // o < 0: before this range
       hz_i_++;
-      next_hz_ = hz_i_ < nhz_ ? (int)hz_[hz_i_] : 0x7fffffff;
+      next_hz_ = hz_i_ < nhz_ ? (int)hz_[hz_i_] - hz_bias_ : 0x7fffffff;
     }
 
     uint32_t f[NT];
@@ -242,6 +245,7 @@ class StreamParser {
   int lo_, len_;
   const uint32_t* __restrict__ hz_;
   int nhz_;
+  int hz_bias_;
   int lane_;
   int4 cur_[NV], next_[NV];
   int hz_i_ = 0, next_hz_;

@@ -157,13 +157,27 @@ def test_decode_coeffs(case, request):
             assert rel(want2, want) <= 1e-14
 
 
+# The spatial decode runs its inverse DCT on MFMA with BF16 hi + lo split
+# products (FP32 accumulation): per-column relative L2 error against the FP64
+# oracle is bounded by SPATIAL_TOL.
+SPATIAL_TOL = 3e-5
+
+
 @pytest.mark.parametrize("case", CASES)
 def test_decode_columns(case, request):
     ob, gb = request.getfixturevalue(case)
     cols = gb.decode_columns().numpy()
+    errs = []
     for c in range(ob.r):
         want = ob.decode_vector_field(c)
-        assert rel(cols[c], want) <= TOL, (c, rel(cols[c], want))
+        errs.append(rel(cols[c], want))
+        assert errs[-1] <= SPATIAL_TOL, (c, errs[-1])
+    print(f"[{case}] decode_columns per-column rel-L2: max {max(errs):.2e} mean {np.mean(errs):.2e}")
+    # a column range that starts inside a group of columns
+    if ob.r > 2:
+        part = gb.decode_columns(1, ob.r - 2).numpy()
+        for c in range(1, ob.r - 1):
+            assert np.array_equal(part[c - 1], cols[c]), c
 
 
 def _fields(ob, seed):
