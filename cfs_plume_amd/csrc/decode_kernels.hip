@@ -194,6 +194,46 @@ __host__ inline size_t spatial_lds_bytes(int nk) {
   return spatial_shared_lds_bytes(nk) + 3 * spatial_comp_lds_bytes() + (size_t)2 * BC * 3 * 4;
 }
 
+// One work item's column group and this wave's segment parameters (lane < G).
+struct SpatialItem {
+  int b, c0, G;
+  uint32_t lk, off;
+  float sv;
+};
+
+__device__ __forceinline__ void spatial_load(const DevBasis& B, const DecodeArgs& a, int k, long item,
+                                             SpatialItem& it) {
+  const int grp = (int)(item / B.nB);
+  it.b = (int)(item % B.nB);
+  it.c0 = a.col0 + grp * SP_G;
+  it.G = min(SP_G, a.col0 + a.ncol - it.c0);
+  const size_t si0 = ((size_t)k * B.nB + it.b) * B.r + it.c0;
+  const int lane = lane_id();
+  it.lk = 0;
+  it.off = 0;
+  it.sv = 0.0f;
+  if (lane < it.G) {
+    it.lk = B.seg_lk[si0 + lane];
+    it.off = B.seg_off[si0 + lane];
+    it.sv = B.seg_sinv[si0 + lane];
+  }
+}
+// token range [lo, hi) of the group inside its block stream
+__device__ __forceinline__ int spatial_lo(const SpatialItem& it) { return readlane((int)it.off, 0); }
+__device__ __forceinline__ int spatial_hi(const SpatialItem& it) {
+  return readlane((int)it.off, it.G - 1) + (int)(readlane(it.lk, it.G - 1) & 0xffffu);
+}
+__device__ __forceinline__ StreamParser<SP_NT> spatial_parser(const DevBasis& B, int k, const SpatialItem& it) {
+  const size_t kb = (size_t)k * B.nB + it.b;
+  const int lo = spatial_lo(it), hi = spatial_hi(it);
+  const int lo_al = lo & ~3;
+  const uint32_t h0 = B.hz_start[kb], h1 = B.hz_start[kb + 1];
+  return StreamParser<SP_NT>(B.tokens + B.blk_base[kb] + lo_al, lo - lo_al, hi - lo_al, B.hz + h0, (int)(h1 - h0),
+                             lo_al);
+}
+
+// Persistent: workgroups stride over the work items; the next item's
+// parameters and first token chunks are loaded while this one transforms.
 __global__ void __launch_bounds__(3 * WAVE)
 spatial_kernel(DevBasis B, DecodeArgs a, const float* __restrict__ g_damp, const Tables* __restrict__ g_tab) {
   extern __shared__ __align__(16) char lds[];
@@ -208,86 +248,15 @@ spatial_kernel(DevBasis B, DecodeArgs a, const float* __restrict__ g_damp, const
   int2* prm = reinterpret_cast<int2*>(cbase + (size_t)SP_G * BC * 4);   // [SP_G + 1] {row byte offset, bits(1/s)}
   float* tile = reinterpret_cast<float*>(lds + spatial_shared_lds_bytes(nk) + 3 * spatial_comp_lds_bytes());
 
+  const long total = (long)((a.ncol + SP_G - 1) / SP_G) * B.nB;
+  long item = blockIdx.x;
+  SpatialItem cur;
+  spatial_load(B, a, k, item, cur);
+  StreamParser<SP_NT> parser = spatial_parser(B, k, cur);
+
   for (int i = threadIdx.x; i < nk * NDIAG; i += blockDim.x) damp[i] = g_damp[i];
   for (int i = threadIdx.x; i < BC; i += blockDim.x)
     jinfo[i] = (uint32_t)g_tab->rev[i] | ((uint32_t)(4 * g_tab->dz[i]) << 16);
-
-  const int r = B.r;
-  const int ngroups = (a.ncol + SP_G - 1) / SP_G;
-  const int grp = (int)(blockIdx.x / (unsigned)B.nB);
-  const int b = (int)(blockIdx.x % (unsigned)B.nB);
-  const int c0 = a.col0 + grp * SP_G;
-  const int G = min(SP_G, a.col0 + a.ncol - c0);
-  const size_t kb = (size_t)k * B.nB + b;
-  const size_t si0 = kb * r + c0;
-  (void)ngroups;
-
-  // column c0 + lane's segment: parameters and its token range in the block stream
-  uint32_t lk = 0, off = 0;
-  float sv = 0.0f;
-  if (lane < G) {
-    lk = B.seg_lk[si0 + lane];
-    off = B.seg_off[si0 + lane];
-    sv = B.seg_sinv[si0 + lane];
-  }
-  const int lo = readlane((int)off, 0);
-  const int hi = readlane((int)off, G - 1) + (int)(readlane(lk, G - 1) & 0xffffu);
-  if (lane <= G) prm[lane] = lane < G ? make_int2((int)(lk >> 16) * (NDIAG * 4), __float_as_int(sv)) : make_int2(0, 0);
-  {
-    float4* x4 = reinterpret_cast<float4*>(X);
-    for (int i = lane; i < SP_G * BC / 4; i += WAVE) x4[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  }
-  __syncthreads();  // tables and this wave's buffers
-
-  // parse the group's tokens as one stream; tokens outside [lo, hi) are ignored
-  const int lo_al = lo & ~3;
-  const int32_t* tok = B.tokens + B.blk_base[kb] + lo_al;
-  const uint32_t h0 = B.hz_start[kb], h1 = B.hz_start[kb + 1];
-  StreamParser<SP_NT> parser(tok, lo - lo_al, hi - lo_al, B.hz + h0, (int)(h1 - h0), lo_al);
-  // long runs of columns c0 .. c0+G-1: [seg_lr[si0], seg_lr[si0 + G]) (seg_lr has one extra entry)
-  uint32_t li = B.seg_lr[si0];
-  const uint32_t l1 = B.seg_lr[si0 + G];
-  int next_lr = li < l1 ? (int)B.lr[li] - lo_al : 0x7fffffff;
-  for (int base = 0; base < hi - lo_al; base += 64 * SP_NT) {
-    Slots<SP_NT> sl;
-    if (parser.plain(base))
-      parser.template parse_slots<true>(base, sl);
-    else
-      parser.template parse_slots<false>(base, sl);
-#pragma unroll
-    for (int i = 0; i < SP_NT; i++) {
-      if (sl.val[i] != 0) {
-        const int2 pr = prm[sl.pos[i] >> 9];
-        const uint32_t ji = jinfo[sl.pos[i] & (BC - 1)];
-        const float dm = *reinterpret_cast<const float*>(lds + (pr.x + (int)(ji >> 16)));
-        X[(sl.pos[i] >> 9) * BC + (int)(ji & 0xffffu)] = (float)sl.val[i] * (dm * __int_as_float(pr.y));
-      }
-    }
-    // values P+3 .. P+L-1 of nonzero runs with L >= 4
-    while (next_lr < min(base + 64 * SP_NT, hi - lo_al)) {
-      const int o = next_lr - base;
-      const int ol = o / SP_NT, os = o % SP_NT;
-      int pv = 0;
-#pragma unroll
-      for (int i = 0; i < SP_NT; i++)
-        if (i == os) pv = sl.pos[i];
-      pv = readlane(pv, ol);
-      const int Lv = tok[next_lr];
-      const int vv = tok[next_lr - 1];
-      const int2 pr = prm[pv >> 9];
-      for (int m = lane; m < Lv - 3; m += WAVE) {
-        const int p = pv + 1 + m;
-        const uint32_t ji = jinfo[p & (BC - 1)];
-        const float dm = *reinterpret_cast<const float*>(lds + (pr.x + (int)(ji >> 16)));
-        X[(p >> 9) * BC + (int)(ji & 0xffffu)] = (float)vv * (dm * __int_as_float(pr.y));
-      }
-      li++;
-      next_lr = li < l1 ? (int)B.lr[li] - lo_al : 0x7fffffff;
-    }
-  }
-  if (!parser.complete(G) && lane == 0) atomicOr(B.err, (uint32_t)ERR_INCOMPLETE);
-  wave_lds_sync();
-
   // constant MFMA operands of this lane
   bf16x4_t b1h, b1l;
   bf16x8_t a2h[4][2], a2l[4][2];
@@ -306,75 +275,151 @@ spatial_kernel(DevBasis B, DecodeArgs a, const float* __restrict__ g_damp, const
         a2l[t][ks] = as_bf16x8(*reinterpret_cast<const int4*>(mf.s2a[1][t][ks][lane]));
       }
   }
-
   const int g = lane >> 4, n = lane & 15;
-  const int bx = b % B.nbx, by = (b / B.nbx) % B.nby, bz = b / (B.nbx * B.nby);
   const size_t col_stride = (size_t)3 * B.X * B.Y * B.Z;
-  for (int s0 = 0; s0 < G; s0 += 2) {
-    // x pass: lane (m = n, g) reads row 16t + n of block s0 + (g >> 1), u = 4 (g & 1) .. +3
-    const float* xs = X + (s0 + (g >> 1)) * BC + 4 * (g & 1);
-    f32x4_t d1[4];
-#pragma unroll
-    for (int t = 0; t < 4; t++) {
-      const float4 v = *reinterpret_cast<const float4*>(xs + (16 * t + n) * BS);
-      short h0, h1, h2, h3, l0, l1, l2, l3;
-      split_bf16(v.x, h0, l0);
-      split_bf16(v.y, h1, l1);
-      split_bf16(v.z, h2, l2);
-      split_bf16(v.w, h3, l3);
-      const bf16x4_t ah = {h0, h1, h2, h3}, al = {l0, l1, l2, l3};
-      f32x4_t c = {0.0f, 0.0f, 0.0f, 0.0f};
-      c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, b1h, c, 0, 0, 0);
-      c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, b1l, c, 0, 0, 0);
-      c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, b1h, c, 0, 0, 0);
-      d1[t] = c;
-    }
-    // (y, z) pass: B operand straight from the x pass's accumulators
-    bf16x8_t b2h[2], b2l[2];
-#pragma unroll
-    for (int ks = 0; ks < 2; ks++)
-#pragma unroll
-      for (int i = 0; i < 8; i++) {
-        short h, l;
-        split_bf16(i < 4 ? d1[2 * ks][i] : d1[2 * ks + 1][i - 4], h, l);
-        __bf16 hb, lb;
-        __builtin_memcpy(&hb, &h, 2);
-        __builtin_memcpy(&lb, &l, 2);
-        b2h[ks][i] = hb;
-        b2l[ks][i] = lb;
-      }
-    // lane column n = (block s0 + n / 8, x = n % 8), rows (z, y) = 16 t + 4 g + i;
-    // tile[(block, cell x + 8y + 64z)][k]
-    float* tp = tile + ((size_t)(n >> 3) * BC + (n & 7)) * 3 + k;
-#pragma unroll
-    for (int t = 0; t < 4; t++) {
-      f32x4_t c = {0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-      for (int ks = 0; ks < 2; ks++) {
-        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2h[t][ks], b2h[ks], c, 0, 0, 0);
-        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2h[t][ks], b2l[ks], c, 0, 0, 0);
-        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2l[t][ks], b2h[ks], c, 0, 0, 0);
-      }
-#pragma unroll
-      for (int i = 0; i < 4; i++) tp[(16 * t + 4 * g + i) * BS * 3] = c[i];
-    }
-    __syncthreads();  // the three components of the pair's cells are in the tile
-    // rows of 8 cells x 3 components = 24 contiguous floats per (block, z, y)
+  bool complete = true;
+
+  for (; item < total; item += gridDim.x) {
+    const int b = cur.b, c0 = cur.c0, G = cur.G;
+    const size_t kb = (size_t)k * B.nB + b;
+    const size_t si0 = kb * B.r + c0;
+    const int lo = spatial_lo(cur), hi = spatial_hi(cur);
+    const int lo_al = lo & ~3;
+    if (lane <= G)
+      prm[lane] = lane < G ? make_int2((int)(cur.lk >> 16) * (NDIAG * 4), __float_as_int(cur.sv)) : make_int2(0, 0);
     {
-      const int f24 = threadIdx.x % 24;
-      const int x = f24 / 3;
-      const int gx = bx * BS + x;
-      for (int row = threadIdx.x / 24; row < 2 * 64; row += (3 * WAVE) / 24) {
-        const int sb = row >> 6, q = row & 63;
-        const int gy = by * BS + (q & 7), gz = bz * BS + (q >> 3);
-        if (s0 + sb < G && gx < B.X && gy < B.Y && gz < B.Z) {
-          float* o = reinterpret_cast<float*>(a.out) + (size_t)(c0 + s0 + sb - a.col0) * col_stride;
-          o[3 * (gx + B.X * (gy + B.Y * gz)) + f24 % 3] = tile[(sb * BC + q * BS) * 3 + f24];
+      float4* x4 = reinterpret_cast<float4*>(X);
+      for (int i = lane; i < SP_G * BC / 4; i += WAVE) x4[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+    __syncthreads();  // tables (first item) and this wave's buffers
+    const long nitem = item + gridDim.x;
+    SpatialItem nx = cur;
+    if (nitem < total) spatial_load(B, a, k, nitem, nx);
+
+    // parse the group's tokens as one stream; tokens outside [lo, hi) are ignored
+    const int32_t* tok = B.tokens + B.blk_base[kb] + lo_al;
+    // long runs of columns c0 .. c0+G-1: [seg_lr[si0], seg_lr[si0 + G]) (seg_lr has one extra entry)
+    uint32_t li = B.seg_lr[si0];
+    const uint32_t l1 = B.seg_lr[si0 + G];
+    int next_lr = li < l1 ? (int)B.lr[li] - lo_al : 0x7fffffff;
+    for (int base = 0; base < hi - lo_al; base += 64 * SP_NT) {
+      Slots<SP_NT> sl;
+      if (parser.plain(base))
+        parser.template parse_slots<true>(base, sl);
+      else
+        parser.template parse_slots<false>(base, sl);
+#pragma unroll
+      for (int i = 0; i < SP_NT; i++) {
+        if (sl.val[i] != 0) {
+          const int2 pr = prm[sl.pos[i] >> 9];
+          const uint32_t ji = jinfo[sl.pos[i] & (BC - 1)];
+          const float dm = *reinterpret_cast<const float*>(lds + (pr.x + (int)(ji >> 16)));
+          X[(sl.pos[i] >> 9) * BC + (int)(ji & 0xffffu)] = (float)sl.val[i] * (dm * __int_as_float(pr.y));
         }
       }
+      // values P+3 .. P+L-1 of nonzero runs with L >= 4
+      while (next_lr < min(base + 64 * SP_NT, hi - lo_al)) {
+        const int o = next_lr - base;
+        const int ol = o / SP_NT, os = o % SP_NT;
+        int pv = 0;
+#pragma unroll
+        for (int i = 0; i < SP_NT; i++)
+          if (i == os) pv = sl.pos[i];
+        pv = readlane(pv, ol);
+        const int Lv = tok[next_lr];
+        const int vv = tok[next_lr - 1];
+        const int2 pr = prm[pv >> 9];
+        for (int m = lane; m < Lv - 3; m += WAVE) {
+          const int p = pv + 1 + m;
+          const uint32_t ji = jinfo[p & (BC - 1)];
+          const float dm = *reinterpret_cast<const float*>(lds + (pr.x + (int)(ji >> 16)));
+          X[(p >> 9) * BC + (int)(ji & 0xffffu)] = (float)vv * (dm * __int_as_float(pr.y));
+        }
+        li++;
+        next_lr = li < l1 ? (int)B.lr[li] - lo_al : 0x7fffffff;
+      }
     }
-    __syncthreads();  // the tile is free for the next pair
+    complete = complete && parser.complete(G);
+    // the next item's first token chunks, in flight during this item's transforms
+    if (nitem < total) parser = spatial_parser(B, k, nx);
+    wave_lds_sync();
+
+    const int bx = b % B.nbx, by = (b / B.nbx) % B.nby, bz = b / (B.nbx * B.nby);
+    for (int s0 = 0; s0 < G; s0 += 2) {
+      // x pass: lane (m = n, g) reads row 16t + n of block s0 + (g >> 1), u = 4 (g & 1) .. +3
+      const float* xs = X + (s0 + (g >> 1)) * BC + 4 * (g & 1);
+      f32x4_t d1[4];
+#pragma unroll
+      for (int t = 0; t < 4; t++) {
+        const float4 v = *reinterpret_cast<const float4*>(xs + (16 * t + n) * BS);
+        short h0, h1, h2, h3, l0, l1, l2, l3;
+        split_bf16(v.x, h0, l0);
+        split_bf16(v.y, h1, l1);
+        split_bf16(v.z, h2, l2);
+        split_bf16(v.w, h3, l3);
+        const bf16x4_t ah = {h0, h1, h2, h3}, al = {l0, l1, l2, l3};
+        f32x4_t c = {0.0f, 0.0f, 0.0f, 0.0f};
+        c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, b1h, c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, b1l, c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, b1h, c, 0, 0, 0);
+        d1[t] = c;
+      }
+      // (y, z) pass: B operand straight from the x pass's accumulators
+      bf16x8_t b2h[2], b2l[2];
+#pragma unroll
+      for (int ks = 0; ks < 2; ks++)
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+          short h, l;
+          split_bf16(i < 4 ? d1[2 * ks][i] : d1[2 * ks + 1][i - 4], h, l);
+          __bf16 hb, lb;
+          __builtin_memcpy(&hb, &h, 2);
+          __builtin_memcpy(&lb, &l, 2);
+          b2h[ks][i] = hb;
+          b2l[ks][i] = lb;
+        }
+      // lane column n = (block s0 + n / 8, x = n % 8), rows (z, y) = 16 t + 4 g + i;
+      // tile[(block, cell x + 8y + 64z)][k]
+      float* tp = tile + ((size_t)(n >> 3) * BC + (n & 7)) * 3 + k;
+#pragma unroll
+      for (int t = 0; t < 4; t++) {
+        f32x4_t c = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int ks = 0; ks < 2; ks++) {
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2h[t][ks], b2h[ks], c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2h[t][ks], b2l[ks], c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2l[t][ks], b2h[ks], c, 0, 0, 0);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++) tp[(16 * t + 4 * g + i) * BS * 3] = c[i];
+      }
+      __syncthreads();  // the three components of the pair's cells are in the tile
+      // rows of 8 cells x 3 components = 24 contiguous floats per (block, z, y):
+      // thread (y, f) = (tid / 24, tid % 24) writes float f of row (z, y) for every z
+      {
+        const int f24 = threadIdx.x % 24, y = threadIdx.x / 24;
+        const int gx = bx * BS + f24 / 3, gy = by * BS + y;
+        const int zmax = min(BS, B.Z - bz * BS);
+        if (gx < B.X && gy < B.Y) {
+          const int cell0 = 3 * (gx + B.X * (gy + B.Y * bz * BS)) + f24 % 3;
+          const int zstep = 3 * B.X * B.Y;
+#pragma unroll
+          for (int sb = 0; sb < 2; sb++) {
+            if (s0 + sb < G) {
+              float* o = reinterpret_cast<float*>(a.out) + (size_t)(c0 + s0 + sb - a.col0) * col_stride + cell0;
+              const float* tr = tile + (sb * BC + y * BS) * 3 + f24;
+#pragma unroll
+              for (int z = 0; z < BS; z++)
+                if (z < zmax) o[z * zstep] = tr[z * 64 * 3];
+            }
+          }
+        }
+      }
+      __syncthreads();  // the tile is free for the next pair
+    }
+    cur = nx;
   }
+  if (!complete && lane == 0) atomicOr(B.err, (uint32_t)ERR_INCOMPLETE);
 }
 
 // Host: the MFMA operands of the inverse DCT in lane order (FP64 closed form
@@ -425,8 +470,13 @@ hipError_t launch_decode(const DevBasis& B, int mode, const DecodeArgs& a, const
     const long items = (long)((a.ncol + SP_G - 1) / SP_G) * B.nB;
     if (items <= 0 || a.k0 != 0 || a.nk3 != 3 || a.b0 != 0 || a.nblk != B.nB) return hipErrorInvalidValue;
     const size_t lds = spatial_lds_bytes(B.nk);
-    const long grid = (long)((a.ncol + SP_G - 1) / SP_G) * B.nB;  // one workgroup per (column group, block)
-    if (grid > 0x7fffffffL) return hipErrorInvalidValue;
+    int dev = 0, cus = 0, per = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, spatial_kernel, 3 * WAVE, lds);
+    if (e != hipSuccess) return e;
+    if (per < 1) return hipErrorInvalidConfiguration;
+    const long grid = std::min<long>(items, (long)cus * per);
     hipLaunchKernelGGL(spatial_kernel, dim3((unsigned)grid), dim3(3 * WAVE), lds, s, B, a, d_damp,
                        reinterpret_cast<const Tables*>(d_tab));
     return hipGetLastError();
