@@ -77,6 +77,7 @@ def main():
     ap.add_argument("--cpu-sample-cols", type=int, default=200,
                     help="columns of the oracle CPU step to time (the full step when >= r)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-spatial", action="store_true", help="skip the spatial decode of all columns")
     ap.add_argument("--threads", type=int, default=16)
     ap.add_argument("--cache", default="", help="directory to keep/reuse the encoded component files")
     ap.add_argument("--mode", default="step", choices=["step", "project", "unproject"],
@@ -196,6 +197,31 @@ def main():
         },
     }
 
+    # Spatial decode of every column (DecodeVectorField for c = 0..r-1): the
+    # inverse DCTs run on MFMA (BF16 hi + lo products, FP32 accumulation).
+    spatial = None
+    if not args.no_spatial:
+        cols = DeviceArray((r, 3 * basis.peeled_cells), np.float32)
+        basis.decode_columns(0, r, cols, stream)
+        stream.synchronize()
+        e0, e1 = Event(), Event()
+        e0.record(stream)
+        for _ in range(3):
+            basis.decode_columns(0, r, cols, stream)
+        e1.record(stream)
+        stream.synchronize()
+        basis.raise_on_stream_errors()
+        ms_dec = e0.elapsed_ms(e1) / 3
+        out_bytes = r * 3 * basis.peeled_cells * 4
+        in_bytes = 4 * sum(tokens) + 24 * 3 * r * basis.nB
+        spatial = {
+            "columns": r, "ms": round(ms_dec, 4), "output_GB": round(out_bytes / 1e9, 3),
+            "GB_per_s": round((out_bytes + in_bytes) / (ms_dec * 1e-3) / 1e9, 1),
+            "inverse_dct": "MFMA bf16 hi+lo split products, fp32 accumulation",
+        }
+        result["spatial_decode"] = spatial
+        log(f"[bench] spatial decode of {r} columns: {ms_dec:.3f} ms ({spatial['GB_per_s']} GB/s read+written)")
+
     if not args.no_cpu:
         ns = min(args.cpu_sample_cols, r)
         sps, t_full, t_sample, q_s, f_s = cpu_baseline(bufs, v, q, r, ns)
@@ -218,6 +244,20 @@ def main():
             "field_rel_l2": float(np.linalg.norm(f_gpu_s - f_s) / np.linalg.norm(f_s)),
             "columns": ns,
         }
+        if spatial is not None:
+            # per-column error of the spatial decode against the FP64 oracle (sampled columns)
+            from oracle import oracle as O
+            from cfs_plume_amd.hip import check, hip
+            ob = O.Basis(buffers=bufs)
+            errs = []
+            for c in sorted({0, r // 3, (2 * r) // 3, r - 1}):
+                got = np.empty(3 * basis.peeled_cells, np.float32)
+                check(hip().hipMemcpy(got.ctypes.data, cols.ptr + c * got.nbytes, got.nbytes, 2), "hipMemcpy D2H")
+                want = ob.decode_vector_field(c)
+                errs.append(float(np.linalg.norm(got - want) / np.linalg.norm(want)))
+            ob.close()
+            spatial["column_rel_l2_max"] = max(errs)
+            spatial["columns_checked"] = 4
     if args.copy_calibration_mb:
         n = args.copy_calibration_mb << 20
         src = DeviceArray.zeros((n,), np.uint8)

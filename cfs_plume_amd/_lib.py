@@ -34,6 +34,15 @@ SIGNATURES = {
     "cfs_decode_block_coeffs": (_i, [_vp, _i, _i, _i, _i, _vp, _vp]),
     "cfs_get_submatrix": (_i, [_vp, _i, _vp, _vp]),
     "cfs_check_stream_errors": (_i, [_vp, ctypes.POINTER(ctypes.c_uint32)]),
+    "cfs_batch_create": (_i, [ctypes.POINTER(_vp), _i, ctypes.POINTER(_vp)]),
+    "cfs_batch_bind": (_i, [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp)]),
+    "cfs_batch_step": (_i, [_vp, _vp]),
+    "cfs_batch_tune": (_i, [_vp, _i, _vp]),
+    "cfs_batch_get_config": (_i, [_vp, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(ctypes.c_float)]),
+    "cfs_batch_set_config": (_i, [_vp, _i, ctypes.c_float]),
+    "cfs_batch_graph_capture": (_i, [_vp, _vp]),
+    "cfs_batch_graph_launch": (_i, [_vp, _vp]),
+    "cfs_batch_destroy": (_i, [_vp]),
     "cfs_encode_matrix": (_i, [_vp, _i, _i, _i, _i, _i, ctypes.c_double, _i, _i, ctypes.POINTER(_vp)]),
     "cfs_encoded_buffer": (_i, [_vp, _i, ctypes.POINTER(_vp), ctypes.POINTER(_sz)]),
     "cfs_encoded_write": (_i, [_vp, _cp]),

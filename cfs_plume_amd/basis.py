@@ -206,3 +206,61 @@ def step(pre: CompressedBasis, fin: CompressedBasis, v_in: DeviceArray, q_out: D
          field: DeviceArray, stream: Optional[Stream] = None):
     """stepPlume's pair: q_out = U_pre^T v_in, field interior = U_fin q_in."""
     _lib.check(_lib.lib().cfs_step(pre._h, fin._h, v_in.ptr, q_out.ptr, q_in.ptr, field.ptr, _sptr(stream)), "step")
+
+
+class BasisBatch:
+    """Several bases stepped in one persistent launch (cfs_batch_*).
+
+    bind() gives every basis its v_in, q_out, q_in and field; step() runs the
+    fused step of all of them; tune() runs the autotuner; capture() records
+    one step into a HIP graph that replay() launches."""
+
+    def __init__(self, bases: Sequence[CompressedBasis]):
+        self.bases = list(bases)
+        arr = (ctypes.c_void_p * len(self.bases))(*[b._h for b in self.bases])
+        h = ctypes.c_void_p()
+        _lib.check(_lib.lib().cfs_batch_create(arr, len(self.bases), ctypes.byref(h)), "cfs_batch_create")
+        self._h = h.value
+        self._bound = None
+
+    def bind(self, v_in, q_out, q_in, field):
+        n = len(self.bases)
+        for b, v, qo, qi, f in zip(self.bases, v_in, q_out, q_in, field):
+            assert v.dtype == np.float32 and v.nbytes == 12 * b.full_cells
+            assert f.dtype == np.float32 and f.nbytes == 12 * b.full_cells
+            assert qo.dtype == np.float64 and qo.nbytes == 8 * b.r and qi.dtype == np.float64 and qi.nbytes == 8 * b.r
+        ptrs = [(ctypes.c_void_p * n)(*[x.ptr for x in arrs]) for arrs in (v_in, q_out, q_in, field)]
+        _lib.check(_lib.lib().cfs_batch_bind(self._h, *ptrs), "cfs_batch_bind")
+        self._bound = (list(v_in), list(q_out), list(q_in), list(field))  # keep the arrays alive
+
+    def step(self, stream: Optional[Stream] = None):
+        _lib.check(_lib.lib().cfs_batch_step(self._h, _sptr(stream)), "cfs_batch_step")
+
+    def tune(self, reps: int = 3, stream: Optional[Stream] = None):
+        _lib.check(_lib.lib().cfs_batch_tune(self._h, reps, _sptr(stream)), "cfs_batch_tune")
+        return self.config()
+
+    def config(self):
+        g, w, iw = ctypes.c_int(), ctypes.c_int(), ctypes.c_float()
+        _lib.check(_lib.lib().cfs_batch_get_config(self._h, ctypes.byref(g), ctypes.byref(w), ctypes.byref(iw)))
+        return {"grid": g.value, "wg_per_cu": w.value, "item_weight": iw.value}
+
+    def set_config(self, wg_per_cu: int, item_weight: float):
+        _lib.check(_lib.lib().cfs_batch_set_config(self._h, wg_per_cu, item_weight), "cfs_batch_set_config")
+
+    def capture(self, stream: Stream):
+        _lib.check(_lib.lib().cfs_batch_graph_capture(self._h, _sptr(stream)), "cfs_batch_graph_capture")
+
+    def replay(self, stream: Stream):
+        _lib.check(_lib.lib().cfs_batch_graph_launch(self._h, _sptr(stream)), "cfs_batch_graph_launch")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.lib().cfs_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -21,7 +21,7 @@ LIB = os.path.join(HERE, "libcfs_plume.so")
 ARCH = "gfx950"
 
 HIP_SOURCES = ["step_kernels.hip", "decode_kernels.hip"]
-HIP_HOST_SOURCES = ["basis.cpp"]
+HIP_HOST_SOURCES = ["basis.cpp", "batch.cpp"]
 CXX_SOURCES = ["encoder.cpp"]
 
 HIPCC_FLAGS = [

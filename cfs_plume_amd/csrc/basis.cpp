@@ -22,6 +22,7 @@
 #include <vector>
 
 #include "../../include/cfs_plume.h"
+#include "basis_internal.h"
 #include "cfs_device.h"
 
 namespace cfs {
@@ -40,10 +41,9 @@ hipError_t launch_decode(const DevBasis& B, int mode, const DecodeArgs& a, const
 
 using namespace cfs;
 
-namespace {
+static thread_local std::string g_err;
 
-thread_local std::string g_err;
-
+namespace cfs {
 int fail(int code, const std::string& msg) {
   g_err = msg;
   return code;
@@ -51,6 +51,9 @@ int fail(int code, const std::string& msg) {
 int hip_fail(hipError_t e, const char* what) {
   return fail(CFS_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
 }
+}  // namespace cfs
+
+namespace {
 
 int pad_of(int n) { return (BS - (n % BS)) % BS; }
 
@@ -218,6 +221,15 @@ struct cfs_basis {
     for (void* p : allocs) (void)hipFree(p);
   }
 };
+
+namespace cfs {
+int basis_device(const cfs_basis* b) { return b->device; }
+const DevBasis& basis_dev(const cfs_basis* b) { return b->dev; }
+const float* basis_damp(const cfs_basis* b) { return b->d_damp; }
+const void* basis_tables(const cfs_basis* b) { return b->d_tab; }
+int basis_num_blocks(const cfs_basis* b) { return b->nB; }
+int64_t basis_total_tokens(const cfs_basis* b) { return b->ntok[0] + b->ntok[1] + b->ntok[2]; }
+}  // namespace cfs
 
 namespace {
 

@@ -63,6 +63,24 @@ struct DevBasis {
   int r, nk;                  // columns, rows of the damping table (kmax - kmin + 1)
 };
 
+struct Tables;
+
+// One basis of a batched step (batch.cpp): its device data, bound fields and
+// its share of the persistent grid.
+struct BatchEntry {
+  DevBasis B;
+  const float* v_in;     // full-grid field projected (FP32)
+  const double* q_in;    // q unprojected
+  float* field;          // full-grid field written (interior cells)
+  double* q_out;         // q projected
+  const float* damp;     // the basis's damping table
+  const Tables* tab;     // the basis's small tables
+  double* partials;      // [wg_count][r] per-workgroup partial q
+  int wg_begin, wg_count;
+  int q_begin;           // first block of this entry in the batched reduce
+  int pad;
+};
+
 // arguments of the random-access decode kernel (decode_kernels.hip)
 struct DecodeArgs {
   int k0, nk3;      // components [k0, k0+nk3)

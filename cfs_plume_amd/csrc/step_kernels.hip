@@ -102,10 +102,12 @@ __device__ __forceinline__ void prefetch_item(const DevBasis& B, const float* __
   if (PROJ) gather_block_rows(v_in, B, (int)p.d2.z, (int)p.d2.y, p.rows);
 }
 
+// The fused step of one basis, run by workgroup `wg` of `nwg` that share it.
 template <int MODE>
-__global__ void __launch_bounds__(WAVES_PER_WG * WAVE, 3)  // <= 168 VGPRs: three waves per SIMD
-step_kernel(DevBasis B, const float* __restrict__ v_in, const double* __restrict__ q_in,
-            float* __restrict__ field_out, const float* __restrict__ g_damp, const Tables* __restrict__ g_tab) {
+__device__ __forceinline__ void step_body(const DevBasis& B, const float* __restrict__ v_in,
+                                          const double* __restrict__ q_in, float* __restrict__ field_out,
+                                          const float* __restrict__ g_damp, const Tables* __restrict__ g_tab,
+                                          double* __restrict__ partials, int wg, int nwg) {
   extern __shared__ __align__(16) char lds[];
   constexpr bool PROJ = (MODE & MODE_PROJECT) != 0;
   constexpr bool UNPROJ = (MODE & MODE_UNPROJECT) != 0;
@@ -119,8 +121,8 @@ step_kernel(DevBasis B, const float* __restrict__ v_in, const double* __restrict
   const WaveLds w = carve_wave(lds + step_shared_lds_bytes(nk) + (size_t)wave * step_wave_lds_bytes(B.r), B.r);
   const int r = B.r;
   const int total = 3 * B.nB;
-  const int nwaves = gridDim.x * WAVES_PER_WG;
-  int item = blockIdx.x * WAVES_PER_WG + wave;
+  const int nwaves = nwg * WAVES_PER_WG;
+  int item = wg * WAVES_PER_WG + wave;
 
   ItemPrefetch nx;
   if (item < total) prefetch_item<PROJ>(B, v_in, item, nx);
@@ -302,9 +304,49 @@ step_kernel(DevBasis B, const float* __restrict__ v_in, const double* __restrict
       double s = 0.0;
       for (int wv = 0; wv < WAVES_PER_WG; wv++)
         s += carve_wave(lds + step_shared_lds_bytes(nk) + (size_t)wv * step_wave_lds_bytes(r), r).qacc[c];
-      B.partials[(size_t)blockIdx.x * r + c] = s;
+      partials[(size_t)wg * r + c] = s;
     }
   }
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(WAVES_PER_WG * WAVE, 3)  // <= 168 VGPRs: three waves per SIMD
+step_kernel(DevBasis B, const float* __restrict__ v_in, const double* __restrict__ q_in,
+            float* __restrict__ field_out, const float* __restrict__ g_damp, const Tables* __restrict__ g_tab) {
+  step_body<MODE>(B, v_in, q_in, field_out, g_damp, g_tab, B.partials, blockIdx.x, gridDim.x);
+}
+
+// Several independent bases in one persistent launch: workgroups
+// [wg_begin, wg_begin + wg_count) of the grid step entry e (entries sorted by
+// wg_begin, contiguous).
+template <int MODE>
+__global__ void __launch_bounds__(WAVES_PER_WG * WAVE, 3)
+batch_step_kernel(const BatchEntry* __restrict__ E, int n) {
+  int e = 0;
+  while (e + 1 < n && (int)blockIdx.x >= E[e + 1].wg_begin) e++;
+  const BatchEntry& be = E[e];
+  const DevBasis B = be.B;
+  step_body<MODE>(B, be.v_in, be.q_in, be.field, be.damp, be.tab, be.partials, (int)blockIdx.x - be.wg_begin,
+                  be.wg_count);
+}
+
+// q of every entry: block (e, c) sums partials[0..wg_count)[c] of entry e.
+__global__ void __launch_bounds__(256) batch_reduce_kernel(const BatchEntry* __restrict__ E, int n) {
+  __shared__ double red[256];
+  int e = 0;
+  while (e + 1 < n && (int)blockIdx.x >= E[e + 1].q_begin) e++;
+  const int c = (int)blockIdx.x - E[e].q_begin;
+  const int r = E[e].B.r;
+  const double* part = E[e].partials;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < E[e].wg_count; i += blockDim.x) s += part[(size_t)i * r + c];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) E[e].q_out[c] = red[0];
 }
 
 // q[c] = sum over workgroups of partials[wg][c], in workgroup order (FP64).
@@ -378,6 +420,42 @@ hipError_t step_occupancy(int mode, int r, int nk, int* wg_per_cu) {
   }
   return e;
 }
+
+hipError_t launch_batch_step(const BatchEntry* d_entries, int n, int mode, int grid, int qrows, size_t lds,
+                             hipStream_t s) {
+  switch (mode) {
+    case MODE_PROJECT:
+      hipLaunchKernelGGL(batch_step_kernel<MODE_PROJECT>, dim3(grid), dim3(WAVES_PER_WG * WAVE), lds, s, d_entries, n);
+      break;
+    case MODE_UNPROJECT:
+      hipLaunchKernelGGL(batch_step_kernel<MODE_UNPROJECT>, dim3(grid), dim3(WAVES_PER_WG * WAVE), lds, s, d_entries,
+                         n);
+      break;
+    default:
+      hipLaunchKernelGGL(batch_step_kernel<MODE_BOTH>, dim3(grid), dim3(WAVES_PER_WG * WAVE), lds, s, d_entries, n);
+      break;
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || !(mode & MODE_PROJECT)) return e;
+  hipLaunchKernelGGL(batch_reduce_kernel, dim3(qrows), dim3(256), 0, s, d_entries, n);
+  return hipGetLastError();
+}
+
+hipError_t batch_occupancy(int mode, size_t lds, int* wg_per_cu) {
+  switch (mode) {
+    case MODE_PROJECT:
+      return hipOccupancyMaxActiveBlocksPerMultiprocessor(wg_per_cu, batch_step_kernel<MODE_PROJECT>,
+                                                          WAVES_PER_WG * WAVE, lds);
+    case MODE_UNPROJECT:
+      return hipOccupancyMaxActiveBlocksPerMultiprocessor(wg_per_cu, batch_step_kernel<MODE_UNPROJECT>,
+                                                          WAVES_PER_WG * WAVE, lds);
+    default:
+      return hipOccupancyMaxActiveBlocksPerMultiprocessor(wg_per_cu, batch_step_kernel<MODE_BOTH>,
+                                                          WAVES_PER_WG * WAVE, lds);
+  }
+}
+
+size_t step_lds_bytes_for(int r, int nk) { return step_lds_bytes(r, nk); }
 
 size_t tables_bytes() { return sizeof(Tables); }
 

@@ -112,6 +112,33 @@ int cfs_get_submatrix(cfs_basis* basis, int start_row, double* out, void* stream
  * clears them. */
 int cfs_check_stream_errors(cfs_basis* basis, uint32_t* bits);
 
+/* ---- batched step: several bases, one persistent launch -------------------
+ * No counterpart in the reference, which steps one basis pair per call
+ * (stepPlume, SUBSPACE_FLUID_3D_COMPRESSED_EIGEN.cpp:453-501): a batch steps
+ * independent bases of different grids and column counts together.  Each
+ * basis b gets q_out[b] = U_b^T v_in[b] and the interior of field[b] = U_b q_in[b]
+ * (the fused cfs_step of every basis), in one launch whose workgroups are
+ * split among the bases by work, plus one launch reducing every q.  The
+ * bases must live on one device and outlive the batch; calls on one batch
+ * must not overlap. */
+typedef struct cfs_batch cfs_batch;
+int cfs_batch_create(cfs_basis* const* bases, int n, cfs_batch** out);
+/* Binds the device arrays of every basis (host arrays of n device pointers). */
+int cfs_batch_bind(cfs_batch* batch, const float* const* v_in, double* const* q_out, const double* const* q_in,
+                   float* const* field);
+int cfs_batch_step(cfs_batch* batch, void* stream);
+/* Autotuner: times every launch shape (workgroups per CU x work-split weight)
+ * `reps` times on the bound arrays and keeps the fastest. */
+int cfs_batch_tune(cfs_batch* batch, int reps, void* stream);
+int cfs_batch_get_config(const cfs_batch* batch, int* grid, int* wg_per_cu, float* item_weight);
+int cfs_batch_set_config(cfs_batch* batch, int wg_per_cu, float item_weight);
+/* Captures one whole batched step on `stream` (not the default stream) into
+ * a HIP graph held by the batch; cfs_batch_graph_launch replays it.  Binding,
+ * tuning or setting the configuration drops the graph. */
+int cfs_batch_graph_capture(cfs_batch* batch, void* stream);
+int cfs_batch_graph_launch(cfs_batch* batch, void* stream);
+int cfs_batch_destroy(cfs_batch* batch);
+
 /* ---- encoder (host) ---------------------------------------------------------
  * CompressAndWriteMatrixComponents (COMPRESSION.cpp:2271-2359) with the SVD
  * transform off, as on the live path (:2312).  U is FP64 column-major:
