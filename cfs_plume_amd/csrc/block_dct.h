@@ -1,4 +1,5 @@
-// block_dct.h -- shared tables and the 8x8x8 block DCT helpers (VALU FP32).
+// block_dct.h -- shared tables and the forward 8x8x8 block DCT (VALU FP32);
+// the inverse DCT runs on MFMA (mfma_idct.h).
 //
 // Orthonormal DCT-II / DCT-III along x, y, z, the closed form of the
 // reference's FFTW REDFT10/REDFT01 plus unitary normalisation
@@ -50,21 +51,6 @@ __device__ __forceinline__ void dct8_fwd(const float a[BS], float X[BS]) {
   X[3] = DC3 * d0 - DC7 * d1 - DC1 * d2 - DC5 * d3;
   X[5] = DC5 * d0 - DC1 * d1 + DC7 * d2 + DC3 * d3;
   X[7] = DC7 * d0 - DC5 * d1 + DC3 * d2 - DC1 * d3;
-}
-
-// a = C^T X (DCT-III, the inverse of dct8_fwd).
-__device__ __forceinline__ void dct8_inv(const float X[BS], float a[BS]) {
-  const float E0 = DC4 * (X[0] + X[4]), E1 = DC4 * (X[0] - X[4]);
-  const float P = DC2 * X[2] + DC6 * X[6], Q = DC6 * X[2] - DC2 * X[6];
-  const float e0 = E0 + P, e3 = E0 - P, e1 = E1 + Q, e2 = E1 - Q;
-  const float o0 = DC1 * X[1] + DC3 * X[3] + DC5 * X[5] + DC7 * X[7];
-  const float o1 = DC3 * X[1] - DC7 * X[3] - DC1 * X[5] - DC5 * X[7];
-  const float o2 = DC5 * X[1] - DC1 * X[3] + DC7 * X[5] + DC3 * X[7];
-  const float o3 = DC7 * X[1] - DC5 * X[3] + DC3 * X[5] - DC1 * X[7];
-  a[0] = e0 + o0; a[7] = e0 - o0;
-  a[1] = e1 + o1; a[6] = e1 - o1;
-  a[2] = e2 + o2; a[5] = e2 - o2;
-  a[3] = e3 + o3; a[4] = e3 - o3;
 }
 
 // Offset (in floats) of component k of full-grid cell (gx+1, gy+1, gz+1)
@@ -133,55 +119,6 @@ __device__ void forward_dct_block(const float* __restrict__ v, const DevBasis& B
   float a[BS];
   gather_block_rows(v, B, k, b, a);
   forward_dct_rows<STRIDE>(a, zig, scr, vz);
-}
-
-// Inverse orthonormal 3D DCT of the zigzag-ordered fixed-point accumulator
-// `az` (value = az * unit) and scatter of the interior cells of component k
-// into the full-grid field (UnitaryBlockDCTEigen(-1) + AssimilateBlocksEigen
-// + subfield + setWithPeeled, COMPRESSION.cpp:2880-2898,
-// VECTOR3_FIELD_3D.cpp:3415-3430).  Leaves az zero.
-__device__ void inverse_dct_block(float* __restrict__ field, const DevBasis& B, int k, int b, const uint16_t* zig,
-                                  int* az, float unit, float* scr) {
-  const int lane = lane_id();
-  float X[BS], a[BS];
-  // gather + pass along z: lane = (u, v)
-  {
-    const int u = lane & 7, vv = lane >> 3;
-#pragma unroll
-    for (int w = 0; w < BS; w++) X[w] = (float)az[zig[u + 8 * vv + 64 * w]] * unit;
-    dct8_inv(X, a);
-#pragma unroll
-    for (int z = 0; z < BS; z++) scr[z * 64 + vv * 8 + u] = a[z];
-  }
-  wave_lds_sync();
-  for (int i = lane; i < BC; i += WAVE) az[i] = 0;
-  // pass along y: lane = (u, z)
-  {
-    const int u = lane & 7, z = lane >> 3;
-#pragma unroll
-    for (int vv = 0; vv < BS; vv++) X[vv] = scr[z * 64 + vv * 8 + u];
-    wave_lds_sync();
-    dct8_inv(X, a);
-#pragma unroll
-    for (int y = 0; y < BS; y++) scr[z * 64 + y * 8 + u] = a[y];
-  }
-  wave_lds_sync();
-  // pass along x: lane = (y, z); write interior cells
-  {
-    const int bx = b % B.nbx, by = (b / B.nbx) % B.nby, bz = b / (B.nbx * B.nby);
-    const int y = lane & 7, z = lane >> 3;
-    const int gy = by * BS + y, gz = bz * BS + z;
-#pragma unroll
-    for (int u = 0; u < BS; u++) X[u] = scr[z * 64 + y * 8 + u];
-    dct8_inv(X, a);
-    if (gy < B.Y && gz < B.Z) {
-      float* row = field + field_offset(B, bx * BS, gy, gz, k);
-#pragma unroll
-      for (int x = 0; x < BS; x++)
-        if (bx * BS + x < B.X) row[3 * x] = a[x];
-    }
-  }
-  wave_lds_sync();
 }
 
 }  // namespace cfs

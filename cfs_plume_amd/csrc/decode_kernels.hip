@@ -17,6 +17,7 @@
 
 #include "block_dct.h"
 #include "cfs_device.h"
+#include "mfma_idct.h"
 #include "stream_decode.h"
 #include "wave_ops.h"
 
@@ -163,29 +164,8 @@ decode_kernel(DevBasis B, DecodeArgs a, const float* __restrict__ g_damp, const 
 // keeps hi*hi + hi*lo + lo*hi.  The three components meet in an LDS tile of
 // interleaved cells, written to the column as contiguous 96-byte rows.
 // ---------------------------------------------------------------------------
-typedef short bf16x4_t __attribute__((ext_vector_type(4)));
-typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
-typedef float f32x4_t __attribute__((ext_vector_type(4)));
-
 constexpr int SP_G = 4;    // columns per work item
 constexpr int SP_NT = 4;   // tokens per lane per chunk of the parser
-
-__device__ __forceinline__ short bf16_bits(__bf16 h) {
-  short v;
-  __builtin_memcpy(&v, &h, 2);
-  return v;
-}
-// x = hi + lo, both BF16 (round to nearest even)
-__device__ __forceinline__ void split_bf16(float x, short& hi, short& lo) {
-  const __bf16 h = (__bf16)x;
-  hi = bf16_bits(h);
-  lo = bf16_bits((__bf16)(x - (float)h));
-}
-__device__ __forceinline__ bf16x8_t as_bf16x8(const int4& v) {
-  bf16x8_t r;
-  __builtin_memcpy(&r, &v, 16);
-  return r;
-}
 
 // LDS: damping rows | jinfo[512] | per component: X[SP_G][512] + prm[SP_G+1] | cell tile [2][512][3]
 __host__ __device__ inline size_t spatial_shared_lds_bytes(int nk) { return ((size_t)nk * NDIAG * 4 + 15) / 16 * 16 + BC * 4; }
@@ -258,23 +238,8 @@ spatial_kernel(DevBasis B, DecodeArgs a, const float* __restrict__ g_damp, const
   for (int i = threadIdx.x; i < BC; i += blockDim.x)
     jinfo[i] = (uint32_t)g_tab->rev[i] | ((uint32_t)(4 * g_tab->dz[i]) << 16);
   // constant MFMA operands of this lane
-  bf16x4_t b1h, b1l;
-  bf16x8_t a2h[4][2], a2l[4][2];
-  {
-    const MfmaFrags& mf = g_tab->mf;
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-      b1h[i] = (short)mf.s1b[0][lane][i];
-      b1l[i] = (short)mf.s1b[1][lane][i];
-    }
-#pragma unroll
-    for (int t = 0; t < 4; t++)
-#pragma unroll
-      for (int ks = 0; ks < 2; ks++) {
-        a2h[t][ks] = as_bf16x8(*reinterpret_cast<const int4*>(mf.s2a[0][t][ks][lane]));
-        a2l[t][ks] = as_bf16x8(*reinterpret_cast<const int4*>(mf.s2a[1][t][ks][lane]));
-      }
-  }
+  MfmaConsts mc;
+  load_mfma_consts(g_tab->mf, lane, mc);
   const int g = lane >> 4, n = lane & 15;
   const size_t col_stride = (size_t)3 * B.X * B.Y * B.Z;
   bool complete = true;
@@ -346,53 +311,20 @@ spatial_kernel(DevBasis B, DecodeArgs a, const float* __restrict__ g_damp, const
 
     const int bx = b % B.nbx, by = (b / B.nbx) % B.nby, bz = b / (B.nbx * B.nby);
     for (int s0 = 0; s0 < G; s0 += 2) {
-      // x pass: lane (m = n, g) reads row 16t + n of block s0 + (g >> 1), u = 4 (g & 1) .. +3
+      // lane (n, g) reads row 16 t + n of block s0 + (g >> 1), u = 4 (g & 1) .. +3
       const float* xs = X + (s0 + (g >> 1)) * BC + 4 * (g & 1);
-      f32x4_t d1[4];
+      float4 in[4];
 #pragma unroll
-      for (int t = 0; t < 4; t++) {
-        const float4 v = *reinterpret_cast<const float4*>(xs + (16 * t + n) * BS);
-        short h0, h1, h2, h3, l0, l1, l2, l3;
-        split_bf16(v.x, h0, l0);
-        split_bf16(v.y, h1, l1);
-        split_bf16(v.z, h2, l2);
-        split_bf16(v.w, h3, l3);
-        const bf16x4_t ah = {h0, h1, h2, h3}, al = {l0, l1, l2, l3};
-        f32x4_t c = {0.0f, 0.0f, 0.0f, 0.0f};
-        c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, b1h, c, 0, 0, 0);
-        c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, b1l, c, 0, 0, 0);
-        c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, b1h, c, 0, 0, 0);
-        d1[t] = c;
-      }
-      // (y, z) pass: B operand straight from the x pass's accumulators
-      bf16x8_t b2h[2], b2l[2];
-#pragma unroll
-      for (int ks = 0; ks < 2; ks++)
-#pragma unroll
-        for (int i = 0; i < 8; i++) {
-          short h, l;
-          split_bf16(i < 4 ? d1[2 * ks][i] : d1[2 * ks + 1][i - 4], h, l);
-          __bf16 hb, lb;
-          __builtin_memcpy(&hb, &h, 2);
-          __builtin_memcpy(&lb, &l, 2);
-          b2h[ks][i] = hb;
-          b2l[ks][i] = lb;
-        }
+      for (int t = 0; t < 4; t++) in[t] = *reinterpret_cast<const float4*>(xs + (16 * t + n) * BS);
+      f32x4_t d2[4];
+      idct_pair_mfma(in, mc, d2);
       // lane column n = (block s0 + n / 8, x = n % 8), rows (z, y) = 16 t + 4 g + i;
       // tile[(block, cell x + 8y + 64z)][k]
       float* tp = tile + ((size_t)(n >> 3) * BC + (n & 7)) * 3 + k;
 #pragma unroll
-      for (int t = 0; t < 4; t++) {
-        f32x4_t c = {0.0f, 0.0f, 0.0f, 0.0f};
+      for (int t = 0; t < 4; t++)
 #pragma unroll
-        for (int ks = 0; ks < 2; ks++) {
-          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2h[t][ks], b2h[ks], c, 0, 0, 0);
-          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2h[t][ks], b2l[ks], c, 0, 0, 0);
-          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2l[t][ks], b2h[ks], c, 0, 0, 0);
-        }
-#pragma unroll
-        for (int i = 0; i < 4; i++) tp[(16 * t + 4 * g + i) * BS * 3] = c[i];
-      }
+        for (int i = 0; i < 4; i++) tp[(16 * t + 4 * g + i) * BS * 3] = d2[t][i];
       __syncthreads();  // the three components of the pair's cells are in the tile
       // rows of 8 cells x 3 components = 24 contiguous floats per (block, z, y):
       // thread (y, f) = (tid / 24, tid % 24) writes float f of row (z, y) for every z

@@ -18,6 +18,7 @@
 
 #include "block_dct.h"
 #include "cfs_device.h"
+#include "mfma_idct.h"
 #include "stream_decode.h"
 #include "wave_ops.h"
 
@@ -100,6 +101,64 @@ __device__ __forceinline__ void prefetch_item(const DevBasis& B, const float* __
     p.sv[m] = c < B.r ? B.seg_sinv[i0 + c] : 0.0f;
   }
   if (PROJ) gather_block_rows(v_in, B, (int)p.d2.z, (int)p.d2.y, p.rows);
+}
+
+// Inverse DCT of the zigzag-ordered fixed-point accumulator `az` (value =
+// az * unit) on MFMA (mfma_idct.h: one block in the first half of a pair) and
+// the store of the interior cells of component k into the full-grid field
+// (UnitaryBlockDCTEigen(-1) + AssimilateBlocksEigen + subfield +
+// setWithPeeled, COMPRESSION.cpp:2880-2898, VECTOR3_FIELD_3D.cpp:3415-3430).
+// Leaves az zero.
+__device__ __forceinline__ void inverse_dct_block_mfma(float* __restrict__ field, const DevBasis& B, int k, int b,
+                                                       const uint16_t* zig, int* az, float unit,
+                                                       const MfmaFrags& mf) {
+  const int lane = lane_id();
+  const int g = lane >> 4, n = lane & 15;
+  // lane (n, g < 2) holds row (w, v) = 16 t + n, u = 4 g .. 4 g + 3; lanes g >= 2 (second block) hold zeros
+  float4 in[4];
+#pragma unroll
+  for (int t = 0; t < 4; t++) {
+    float v[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    if (g < 2) {
+      const int nat = (16 * t + n) * BS + 4 * g;
+#pragma unroll
+      for (int i = 0; i < 4; i++) v[i] = (float)az[zig[nat + i]] * unit;
+    }
+    in[t] = make_float4(v[0], v[1], v[2], v[3]);
+  }
+  wave_lds_sync();
+  for (int i = lane; i < BC; i += WAVE) az[i] = 0;
+  bf16x4_t b1h, b1l;
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    b1h[i] = (short)mf.s1b[0][lane][i];
+    b1l[i] = (short)mf.s1b[1][lane][i];
+  }
+  bf16x8_t b2h[2], b2l[2];
+  idct_x_pass(in, b1h, b1l, b2h, b2l);
+  // (y, z) pass tile by tile (one tile's operands live at a time); lane
+  // (n < 8, g) stores cells x = n, (z, y) = 16 t + 4 g + i
+  const int bx = b % B.nbx, by = (b / B.nbx) % B.nby, bz = b / (B.nbx * B.nby);
+  const int gx = bx * BS + n;
+  const bool store = n < BS && gx < B.X;
+#pragma unroll 1
+  for (int t = 0; t < 4; t++) {
+    bf16x8_t a2h[2], a2l[2];
+#pragma unroll
+    for (int ks = 0; ks < 2; ks++) {
+      a2h[ks] = as_bf16x8(*reinterpret_cast<const int4*>(mf.s2a[0][t][ks][lane]));
+      a2l[ks] = as_bf16x8(*reinterpret_cast<const int4*>(mf.s2a[1][t][ks][lane]));
+    }
+    const f32x4_t d = idct_yz_tile(a2h, a2l, b2h, b2l);
+    if (store) {
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int q = 16 * t + 4 * g + i;
+        const int gy = by * BS + (q & 7), gz = bz * BS + (q >> 3);
+        if (gy < B.Y && gz < B.Z) field[field_offset(B, gx, gy, gz, k)] = d[i];
+      }
+    }
+  }
 }
 
 // The fused step of one basis, run by workgroup `wg` of `nwg` that share it.
@@ -292,7 +351,7 @@ __device__ __forceinline__ void step_body(const DevBasis& B, const float* __rest
     if (item + nwaves < total) prefetch_item<PROJ>(B, v_in, item + nwaves, nx);
     wave_lds_sync();
 
-    if (UNPROJ) inverse_dct_block(field_out, B, k, b, zig, w.az, inv_scale, w.pfx);
+    if (UNPROJ) inverse_dct_block_mfma(field_out, B, k, b, zig, w.az, inv_scale, g_tab->mf);
     wave_lds_sync();
   }
   if (!complete && lane == 0) atomicOr(B.err, (uint32_t)ERR_INCOMPLETE);
