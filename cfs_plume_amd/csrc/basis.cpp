@@ -161,6 +161,21 @@ const char* check_segment(const int32_t* t, int len, const double* damp_row, con
   return nullptr;
 }
 
+// Marks the length token of every run of a segment that check_segment
+// accepted: bit (o & 7) of bits[o >> 3] for stream offset o.  Segments of one
+// block stream are marked by one thread, so plain byte writes are safe.
+void mark_len_tokens(const int32_t* t, int len, uint32_t seg_off, uint8_t* bits) {
+  for (int i = 0; i < len;) {
+    if (i + 1 < len && t[i] == t[i + 1]) {
+      const uint32_t o = seg_off + (uint32_t)i + 2;
+      bits[o >> 3] |= (uint8_t)(1u << (o & 7));
+      i += 3;
+    } else {
+      i++;
+    }
+  }
+}
+
 template <class T>
 int dev_upload(T** dst, const T* src, size_t count, size_t* total) {
   const size_t bytes = std::max<size_t>(count * sizeof(T), 4);
@@ -317,6 +332,20 @@ int build_basis(const uint8_t* const bufs[3], const size_t sizes[3], cfs_basis**
       }
     }
   });
+  // LEN-token bitmaps of the fused step: block stream kb owns 64-byte chunks
+  // [mark_chunk[kb], mark_chunk[kb+1]) (one per 512 tokens), plus two guard
+  // chunks for the parser's prefetch.
+  constexpr int MARK_CHUNK = 64 * STEP_TPL;
+  std::vector<uint64_t> mark_chunk((size_t)3 * nB + 1, 0);
+  for (size_t kb = 0; kb < (size_t)3 * nB; kb++)
+    mark_chunk[kb + 1] = mark_chunk[kb] + (blk_len[kb] + MARK_CHUNK - 1) / MARK_CHUNK;
+  if (mark_chunk[(size_t)3 * nB] > 0xffffffffull) return fail(CFS_ERR_FORMAT, "too many token chunks");
+  std::vector<uint8_t> len_bits;
+  try {
+    len_bits.assign((size_t)(mark_chunk[(size_t)3 * nB] + 2) * (MARK_CHUNK / 8), 0);
+  } catch (...) {
+    return fail(CFS_ERR_NOMEM, "host allocation for LEN-token bitmaps failed");
+  }
   // Check every segment; record the "hazard" boundaries of each block stream:
   // a segment ending in a singleton equal to the next segment's first token.
   std::vector<std::vector<uint32_t>> hz_lists((size_t)3 * nB), lr_lists((size_t)3 * nB);
@@ -341,6 +370,7 @@ int build_basis(const uint8_t* const bufs[3], const size_t sizes[3], cfs_basis**
                     std::to_string(c) + ": segment " + why;
           break;
         }
+        mark_len_tokens(stream + seg_off[si], L, seg_off[si], len_bits.data() + mark_chunk[kb] * (MARK_CHUNK / 8));
         if (single && c + 1 < r && stream[seg_off[si] + L - 1] == stream[seg_off[si] + L])
           hz_lists[kb].push_back(seg_off[si] + (uint32_t)L - 1);
         // in the scaled units of the device damping table, rounded up
@@ -411,6 +441,7 @@ int build_basis(const uint8_t* const bufs[3], const size_t sizes[3], cfs_basis**
     std::memcpy(&d[8], &blk_bound[kb], 4);
     d[9] = (uint32_t)b;
     d[10] = (uint32_t)k;
+    d[11] = (uint32_t)mark_chunk[kb];
   }
 
   // damping table (1+d)^(k/4), FP64 -> FP32 (COMPRESSION_DATA.cpp:25-43),
@@ -453,6 +484,7 @@ int build_basis(const uint8_t* const bufs[3], const size_t sizes[3], cfs_basis**
   int32_t* d_order = nullptr;
   double* d_part = nullptr;
   uint8_t* d_tab = nullptr;
+  uint8_t* d_marks = nullptr;
   int st = CFS_OK;
 #define UP(ptr, src, cnt)                        \
   do {                                           \
@@ -477,6 +509,7 @@ int build_basis(const uint8_t* const bufs[3], const size_t sizes[3], cfs_basis**
   UP(d_lr, lr.data(), lr.size());
   UP(d_items, items.data(), items.size());
   UP(d_seglr, seg_lr.data(), seg_lr.size());
+  UP(d_marks, len_bits.data(), len_bits.size());
   const uint32_t zeros[4] = {0, 0, 0, 0};
   UP(d_err, zeros, 1);
   UP(d_cnt, zeros, 2);
@@ -498,6 +531,7 @@ int build_basis(const uint8_t* const bufs[3], const size_t sizes[3], cfs_basis**
   D.lr = d_lr;
   D.items = reinterpret_cast<const uint4*>(d_items);
   D.seg_lr = d_seglr;
+  D.len_bits = d_marks;
   D.err = d_err;
   D.counter = d_cnt;
   D.partials = d_part;

@@ -52,9 +52,11 @@ struct DevBasis {
   const uint32_t* lr_start;   // [3*nB+1] first long run of block stream (k, b) in lr
   const uint32_t* lr;         // offsets of the length tokens of nonzero runs with L >= 4
   const uint32_t* seg_lr;     // [3*nB*r + 1] index in lr of the first long run of each segment (block-major)
+  const uint8_t* len_bits;    // LEN-token bitmap of every block stream, each padded to whole
+                              // 512-token chunks (64 bytes), plus two guard chunks
   const int32_t* order;       // [nB] blocks in descending work order
   const uint4* items;         // [3*nB][3] fused-step items in schedule order:
-                              //   {base lo, base hi, len, k*nB+b}, {hz0, hz1, lr0, lr1}, {bits(blk_bound), b, k, 0}
+                              //   {base lo, base hi, len, k*nB+b}, {hz0, hz1, lr0, lr1}, {bits(blk_bound), b, k, first 64-byte chunk of len_bits}
   uint32_t* err;              // device error bits
   uint32_t* counter;          // [2] spare device words (kept zero)
   double* partials;           // [grid][r] per-workgroup FP64 partial q

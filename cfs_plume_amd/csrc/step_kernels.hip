@@ -213,7 +213,6 @@ __device__ __forceinline__ void step_body(const DevBasis& B, const float* __rest
     const int k = (int)it.d2.z;
     const int len = (int)it.d0.z;
     const uint64_t tbase = (uint64_t)it.d0.x | ((uint64_t)it.d0.y << 32);
-    const uint32_t h0 = it.d1.x, h1 = it.d1.y;
     const uint32_t l0 = it.d1.z, l1 = it.d1.w;
     float inv_scale = 0.0f, S = 0.0f;
     if (UNPROJ) {
@@ -237,7 +236,7 @@ __device__ __forceinline__ void step_body(const DevBasis& B, const float* __rest
     if (PROJ) forward_dct_rows(it.rows, zig, w.pfx, w.vz);
     wave_lds_sync();
 
-    StreamParser<NT> parser(B.tokens + tbase, 0, len, B.hz + h0, (int)(h1 - h0));
+    MarkedStream<NT> parser(B.tokens + tbase, B.len_bits + (size_t)it.d2.w * (CH / 8), len);
     int next_lr = l0 < l1 ? (int)B.lr[l0] : NO_END;
     uint32_t li = l0;
     double run = 0.0;       // project sum of the chunks before this one (wave-uniform)

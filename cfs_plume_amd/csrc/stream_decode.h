@@ -254,4 +254,99 @@ class StreamParser {
   int pos_ = 0;          // output position entering the next chunk
 };
 
+// The fused step's parser for one block stream whose roles were marked at
+// load: bit (o & 7) of byte len_bits[o >> 3] is set iff token o is the length
+// token of a run [v, v, L] (the loader's canonical parse of every segment,
+// RunLengthDecodeBinary, COMPRESSION.cpp:1198-1283).  Tokens and their marks
+// are still read every step; the marks replace the per-token role machine
+// and the hazard list, so a token's role costs one bit test.  Lane l owns
+// tokens [base + 8l, base + 8l + 8) and reads their marks as one byte; chunks
+// are 64-byte aligned in the bitmap.  Same Slots output as
+// StreamParser::parse_slots.
+template <int NT>
+class MarkedStream {
+  static_assert(NT == 8, "one mark byte per lane");
+  static constexpr int NV = NT / 4;
+  static constexpr int CH = 64 * NT;
+
+ public:
+  // tok: 16-byte aligned block stream of `len` tokens followed by at least
+  // two chunks of readable tokens; marks: its LEN bitmap, likewise padded.
+  __device__ MarkedStream(const int32_t* __restrict__ tok, const uint8_t* __restrict__ marks, int len)
+      : tok_(tok), marks_(marks), len_(len) {
+    lane_ = lane_id();
+#pragma unroll
+    for (int q = 0; q < NV; q++) {
+      cur_[q] = *reinterpret_cast<const int4*>(tok_ + NT * lane_ + 4 * q);
+      next_[q] = *reinterpret_cast<const int4*>(tok_ + CH + NT * lane_ + 4 * q);
+    }
+    cur_m_ = marks_[lane_];
+    next_m_ = marks_[CH / 8 + lane_];
+  }
+
+  // True if every token of the chunk at `base` belongs to the stream.
+  __device__ __forceinline__ bool plain(int base) const { return base + CH <= len_; }
+
+  template <bool PLAIN>
+  __device__ __forceinline__ void parse_slots(int base, Slots<NT>& s) {
+    int t[NT];
+#pragma unroll
+    for (int q = 0; q < NV; q++) {
+      t[4 * q] = cur_[q].x;
+      t[4 * q + 1] = cur_[q].y;
+      t[4 * q + 2] = cur_[q].z;
+      t[4 * q + 3] = cur_[q].w;
+    }
+    const uint32_t m = cur_m_;
+    // the token before this lane's first (lane 0: the previous chunk's last)
+    const int tm1 = (int)wave_shr1((uint32_t)t[NT - 1], (uint32_t)prev_last_);
+    prev_last_ = (int)readlane((uint32_t)t[NT - 1], WAVE - 1);
+#pragma unroll
+    for (int q = 0; q < NV; q++) {
+      cur_[q] = next_[q];
+      next_[q] = *reinterpret_cast<const int4*>(tok_ + base + 2 * CH + NT * lane_ + 4 * q);
+    }
+    cur_m_ = next_m_;
+    next_m_ = marks_[(base + 2 * CH) / 8 + lane_];
+    const int tid0 = base + NT * lane_;
+
+    // Counts are selected biased by +2 (VALUE/DUP tokens 3, LEN tokens L,
+    // tokens past the stream 2), which saves a subtraction per token; the
+    // bias is removed once per lane and folded into the per-slot constant.
+    int lane_sum = 0, pre[NT];
+#pragma unroll
+    for (int i = 0; i < NT; i++) {
+      const int prev = i == 0 ? tm1 : t[i - 1];
+      const bool len_tok = (m >> i) & 1u;
+      int cnt2 = len_tok ? t[i] : 3;
+      int val = len_tok ? (t[i] > 2 ? prev : 0) : t[i];
+      if (!PLAIN && tid0 + i >= len_) {
+        cnt2 = 2;
+        val = 0;
+      }
+      s.val[i] = val;
+      pre[i] = lane_sum;
+      lane_sum += cnt2;
+    }
+    const int incl = wave_scan_add(lane_sum);
+    const int p0 = pos_ + incl - lane_sum - 2 * NT * lane_;
+    pos_ += readlane(incl, WAVE - 1) - 2 * NT * WAVE;
+#pragma unroll
+    for (int i = 0; i < NT; i++) s.pos[i] = p0 + pre[i] - 2 * i;
+  }
+
+  // Called after the last chunk: true if the stream produced nseg*512 values.
+  __device__ bool complete(int nseg) const { return pos_ == nseg * BC; }
+
+ private:
+  const int32_t* __restrict__ tok_;
+  const uint8_t* __restrict__ marks_;
+  int len_;
+  int lane_;
+  int4 cur_[NV], next_[NV];
+  uint32_t cur_m_, next_m_;
+  int prev_last_ = 0;
+  int pos_ = 0;
+};
+
 }  // namespace cfs
