@@ -16,6 +16,8 @@
 // across blocks in FP64.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "block_dct.h"
 #include "cfs_device.h"
 #include "mfma_idct.h"
@@ -45,8 +47,10 @@ struct WaveLds {
 __host__ __device__ inline size_t step_wave_lds_bytes(int r) {
   return (size_t)(r + 1) * 16 + 3 * (size_t)BC * 4 + ((size_t)r * 8 + 15) / 16 * 16;
 }
+// Shared part of the workgroup's dynamic LDS: the damping rows (the tables
+// 4*(u+v+w) and zigzag by index are static LDS, 3*512 bytes).
 __host__ __device__ inline size_t step_shared_lds_bytes(int nk) {
-  return ((size_t)nk * NDIAG * 4 + 15) / 16 * 16 + 3 * BC;  // damping rows, 4*(u+v+w), zigzag table
+  return ((size_t)nk * NDIAG * 4 + 15) / 16 * 16;
 }
 __host__ inline size_t step_lds_bytes(int r, int nk) {
   return step_shared_lds_bytes(nk) + WAVES_PER_WG * step_wave_lds_bytes(r);
@@ -172,9 +176,11 @@ __device__ __forceinline__ void step_body(const DevBasis& B, const float* __rest
   constexpr bool UNPROJ = (MODE & MODE_UNPROJECT) != 0;
 
   const int nk = B.nk;
+  // static LDS: its addresses are compile-time constants, so dz4[j] reads
+  // address j + constant with no address arithmetic
+  __shared__ uint8_t dz4[BC];
+  __shared__ uint16_t zig[BC];  // used by the per-block DCTs
   float* damp = reinterpret_cast<float*>(lds);
-  uint8_t* dz4 = reinterpret_cast<uint8_t*>(lds + ((size_t)nk * NDIAG * 4 + 15) / 16 * 16);
-  uint16_t* zig = reinterpret_cast<uint16_t*>(dz4 + BC);  // used by the per-block DCTs
   const int wave = uniform((int)(threadIdx.x / WAVE));
   const int lane = lane_id();
   const WaveLds w = carve_wave(lds + step_shared_lds_bytes(nk) + (size_t)wave * step_wave_lds_bytes(B.r), B.r);
@@ -242,12 +248,15 @@ __device__ __forceinline__ void step_body(const DevBasis& B, const float* __rest
     double run = 0.0;       // project sum of the chunks before this one (wave-uniform)
     double last_end = 0.0;  // running sum at the end of the last closed segment
     int cur = 0;            // first segment not yet closed
-    for (int base = 0; base < len; base += CH) {
+    // chunks alternate between the parser's two token buffers, so the
+    // loads of chunk c+2 are not waited for until chunk c+2
+    auto chunk = [&](const int base, auto par) {
+      constexpr int PAR = decltype(par)::value;
       Slots<NT> sl;
       if (parser.plain(base))
-        parser.template parse_slots<true>(base, sl);
+        parser.template parse_slots<true, PAR>(base, sl);
       else
-        parser.template parse_slots<false>(base, sl);
+        parser.template parse_slots<false, PAR>(base, sl);
       // {bits(1/s), end} of the segments that may close in this chunk
       int2 se = make_int2(0, NO_END);
       if (PROJ) se = *reinterpret_cast<const int2*>(&w.prm[min(cur + lane, r)].z);
@@ -343,6 +352,10 @@ __device__ __forceinline__ void step_body(const DevBasis& B, const float* __rest
         run += (double)readlanef(incl, WAVE - 1);
         wave_lds_sync();
       }
+    };
+    for (int base = 0; base < len; base += 2 * CH) {
+      chunk(base, std::integral_constant<int, 0>());
+      if (base + CH < len) chunk(base + CH, std::integral_constant<int, 1>());
     }
     complete = complete && parser.complete(r);
     // the next item's descriptor, segment parameters and block of v, in

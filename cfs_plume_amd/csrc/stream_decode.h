@@ -277,37 +277,36 @@ class MarkedStream {
     lane_ = lane_id();
 #pragma unroll
     for (int q = 0; q < NV; q++) {
-      cur_[q] = *reinterpret_cast<const int4*>(tok_ + NT * lane_ + 4 * q);
-      next_[q] = *reinterpret_cast<const int4*>(tok_ + CH + NT * lane_ + 4 * q);
+      buf_[0][q] = *reinterpret_cast<const int4*>(tok_ + NT * lane_ + 4 * q);
+      buf_[1][q] = *reinterpret_cast<const int4*>(tok_ + CH + NT * lane_ + 4 * q);
     }
-    cur_m_ = marks_[lane_];
-    next_m_ = marks_[CH / 8 + lane_];
+    m_[0] = marks_[lane_];
+    m_[1] = marks_[CH / 8 + lane_];
   }
 
   // True if every token of the chunk at `base` belongs to the stream.
   __device__ __forceinline__ bool plain(int base) const { return base + CH <= len_; }
 
-  template <bool PLAIN>
+  // PAR: the buffer holding this chunk (chunks alternate, starting with 0).
+  template <bool PLAIN, int PAR>
   __device__ __forceinline__ void parse_slots(int base, Slots<NT>& s) {
     int t[NT];
 #pragma unroll
     for (int q = 0; q < NV; q++) {
-      t[4 * q] = cur_[q].x;
-      t[4 * q + 1] = cur_[q].y;
-      t[4 * q + 2] = cur_[q].z;
-      t[4 * q + 3] = cur_[q].w;
+      t[4 * q] = buf_[PAR][q].x;
+      t[4 * q + 1] = buf_[PAR][q].y;
+      t[4 * q + 2] = buf_[PAR][q].z;
+      t[4 * q + 3] = buf_[PAR][q].w;
     }
-    const uint32_t m = cur_m_;
+    const uint32_t m = m_[PAR];
     // the token before this lane's first (lane 0: the previous chunk's last)
     const int tm1 = (int)wave_shr1((uint32_t)t[NT - 1], (uint32_t)prev_last_);
     prev_last_ = (int)readlane((uint32_t)t[NT - 1], WAVE - 1);
+    // chunk base + 2*CH: a wave-uniform base address plus 32-bit lane offsets
+    const int32_t* tk = tok_ + (base + 2 * CH);
 #pragma unroll
-    for (int q = 0; q < NV; q++) {
-      cur_[q] = next_[q];
-      next_[q] = *reinterpret_cast<const int4*>(tok_ + base + 2 * CH + NT * lane_ + 4 * q);
-    }
-    cur_m_ = next_m_;
-    next_m_ = marks_[(base + 2 * CH) / 8 + lane_];
+    for (int q = 0; q < NV; q++) buf_[PAR][q] = *reinterpret_cast<const int4*>(tk + (uint32_t)(NT * lane_ + 4 * q));
+    m_[PAR] = (marks_ + (base + 2 * CH) / 8)[(uint32_t)lane_];
     const int tid0 = base + NT * lane_;
 
     // Counts are selected biased by +2 (VALUE/DUP tokens 3, LEN tokens L,
@@ -317,7 +316,7 @@ class MarkedStream {
 #pragma unroll
     for (int i = 0; i < NT; i++) {
       const int prev = i == 0 ? tm1 : t[i - 1];
-      const bool len_tok = (m >> i) & 1u;
+      const bool len_tok = __builtin_amdgcn_ubfe(m, i, 1) != 0u;
       int cnt2 = len_tok ? t[i] : 3;
       int val = len_tok ? (t[i] > 2 ? prev : 0) : t[i];
       if (!PLAIN && tid0 + i >= len_) {
@@ -343,8 +342,8 @@ class MarkedStream {
   const uint8_t* __restrict__ marks_;
   int len_;
   int lane_;
-  int4 cur_[NV], next_[NV];
-  uint32_t cur_m_, next_m_;
+  int4 buf_[2][NV];  // tokens of the chunks at base and base + CH
+  uint32_t m_[2];
   int prev_last_ = 0;
   int pos_ = 0;
 };
