@@ -36,6 +36,7 @@ WORKLOADS = {
 }
 DEFAULT = "128x192x128_r200"
 BATCH = "batch16"  # 16 independent bases of seeded sizes in one persistent launch
+CLOSURE = "closure"  # learned advection closure vs the exact reduced-advection step, 48x64x48 r=52
 
 
 def batch_specs(n=16, seed=2024):
@@ -132,6 +133,90 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def run_closure(args):
+    """Learned closure on the 48x64x48, r = 52 basis: exact reduced-plume
+    trajectories (U^T A(U q) on the full grid, the quantity the reference's
+    cubature approximates) of seeded runs train the MLP on the device; then
+    held-out runs are rolled out with the closure and with the exact step, and
+    one step of each is timed (n = 1, HIP events)."""
+    import cfs_plume_amd as C
+    from cfs_plume_amd import closure as CL
+    from cfs_plume_amd import synth
+    from cfs_plume_amd.hip import DeviceArray, Stream
+
+    grid, r, percent = (48, 64, 48), 52, 0.9995
+    n_train, n_held, steps = args.closure_runs, 20, 100
+    bufs = load_or_encode(C, synth, grid, r, percent, 7, "48x64x48_r52", args.cache, args.threads)
+    U = C.CompressedBasis.from_buffers(bufs)
+    plume = C.ReducedPlume(U, dt0=1.0, alpha=0.99, force=25.0)
+    runs = C.ReducedPlume.seeded_runs(n_train + n_held, seed=2026)
+    t0 = time.perf_counter()
+    b_train = plume.forcing(runs[:n_train])
+    traj = plume.rollout_exact(b_train, n_train, steps)
+    t_data = time.perf_counter() - t0
+    q, d = CL.increments(traj, b_train.numpy(), plume.alpha)
+    s_in, s_out = CL.fit_scales(q, d)
+    cl = C.Closure(r, max_batch=1024, seed=5)
+    cl.set_scales(s_in, s_out)
+    t0 = time.perf_counter()
+    losses = cl.train(q * s_in, d / s_out, epochs=args.closure_epochs, batch=512, lr=2e-3, lr_final=1e-5, seed=7,
+                      log=lambda ep, m: log(f"[bench] closure epoch {ep}: mse {m:.3e}") if ep % 20 == 0 else None)
+    t_train = time.perf_counter() - t0
+    log(f"[bench] closure: {n_train} runs x {steps} steps of exact data in {t_data:.1f}s, "
+        f"{args.closure_epochs} epochs in {t_train:.1f}s, final mse {losses[-1]:.3e} (first {losses[0]:.3e})")
+
+    b_held = plume.forcing(runs[n_train:])
+    exact = plume.rollout_exact(b_held, n_held, steps)
+    learned = plume.rollout_closure(cl, b_held, n_held, steps)
+    bh = b_held.numpy()
+    plain = np.zeros_like(exact)
+    for t in range(steps):
+        plain[:, t + 1] = plume.alpha * plain[:, t] + bh
+
+    def drift(a):
+        e = np.linalg.norm(a[:, 1:] - exact[:, 1:], axis=2) / np.linalg.norm(exact[:, 1:], axis=2)
+        return float(e[:, -1].mean()), float(e.max())
+
+    d_cl, d_cl_max = drift(learned)
+    d_pl, _ = drift(plain)
+    checked, falls = plume.rollout_closure_checked(cl, b_held, 0, steps, check_every=10, tol=0.01)
+    d_chk = float(np.linalg.norm(checked[-1] - exact[0, -1]) / np.linalg.norm(exact[0, -1]))
+    log(f"[bench] held-out drift at step {steps}: closure {d_cl:.3e} (max over steps {d_cl_max:.3e}), "
+        f"without advection {d_pl:.3e}; checked every 10 steps: {d_chk:.3e} with {falls} fallbacks")
+
+    # one step of each, n = 1
+    stream = Stream()
+    qa = DeviceArray.from_numpy(exact[0, steps // 2])
+    qb = DeviceArray.zeros((r,), np.float64)
+    reps = args.steps
+    ms_exact = CL.timed(lambda s: C.reduced_advect(U, qa, qb, plume.fa, plume.fb, plume.dt0, plume.alpha,
+                                                   (b_held, 0), s), reps, stream)
+    ms_cl = CL.timed(lambda s: cl.step(qa, qb, 1, plume.alpha, (b_held, 0), s), reps, stream)
+    log(f"[bench] step time: closure {ms_cl * 1e3:.1f} us, exact reduced advection {ms_exact * 1e3:.1f} us "
+        f"-> {ms_exact / ms_cl:.1f}x")
+    result = {
+        "metric": "closure reduced-advection steps/s (r=52 @48x64x48); speedup over the exact reduced-advection step",
+        "value": round(1000.0 / ms_cl, 1), "unit": "steps/s", "n_gpus": 1, "steps": reps, "warmup": 1,
+        "ms_per_step": round(ms_cl, 5), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16 weights, fp32 accumulation", "data": "synthetic",
+        "config": {
+            "workload": f"learned advection closure (MLP {r}-256-256-256-{r}, GELU) on the grid 48x64x48, r={r}; "
+                        f"{n_train} training runs and {n_held} held-out runs of {steps} steps",
+            "grid": list(grid), "r": r, "parallelism": "single GPU",
+        },
+        "closure": {
+            "exact_step_ms": round(ms_exact, 5), "speedup": round(ms_exact / ms_cl, 2),
+            "drift_rel_l2_at_100": round(d_cl, 6), "drift_rel_l2_max": round(d_cl_max, 6),
+            "drift_without_advection": round(d_pl, 6),
+            "checked_every_10": {"drift_at_100": round(d_chk, 6), "fallbacks": falls},
+            "train": {"samples": int(q.shape[0]), "epochs": args.closure_epochs, "batch": 512,
+                      "mse_first": losses[0], "mse_last": losses[-1], "data_s": round(t_data, 2),
+                      "train_s": round(t_train, 2)},
+        },
+    }
+    print(json.dumps(result), flush=True)
+
+
 def traffic_from_profiles(workload: str):
     """HBM bytes per step kernel launch from the committed PMC summary, if any."""
     path = os.path.join(ROOT, "profiles", f"step_kernel_{workload}_pmc.md")
@@ -164,7 +249,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default=DEFAULT, choices=sorted(WORKLOADS) + [BATCH])
+    ap.add_argument("--workload", default=DEFAULT, choices=sorted(WORKLOADS) + [BATCH, CLOSURE])
+    ap.add_argument("--closure-runs", type=int, default=200, help="training runs of the closure workload")
+    ap.add_argument("--closure-epochs", type=int, default=200)
     ap.add_argument("--cpu-sample-cols", type=int, default=200,
                     help="columns of the oracle CPU step to time (the full step when >= r)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -183,6 +270,9 @@ def main():
         sys.exit(2)
     if args.workload == BATCH:
         run_batch(args)
+        return
+    if args.workload == CLOSURE:
+        run_closure(args)
         return
 
     import cfs_plume_amd as C

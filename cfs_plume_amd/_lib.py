@@ -43,6 +43,21 @@ SIGNATURES = {
     "cfs_batch_graph_capture": (_i, [_vp, _vp]),
     "cfs_batch_graph_launch": (_i, [_vp, _vp]),
     "cfs_batch_destroy": (_i, [_vp]),
+    "cfs_reduced_advect": (_i, [_vp, _vp, _vp, _vp, _vp, ctypes.c_float, ctypes.c_double, _vp, _vp]),
+    "cfs_advect_field": (_i, [_vp, _vp, _i, _i, _i, ctypes.c_float, _vp]),
+    "cfs_closure_create": (_i, [_i, _i, _i, ctypes.c_uint64, ctypes.POINTER(_vp)]),
+    "cfs_closure_destroy": (_i, [_vp]),
+    "cfs_closure_get_dims": (_i, [_vp, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i),
+                                  ctypes.POINTER(ctypes.c_int64)]),
+    "cfs_closure_get_params": (_i, [_vp, _vp]),
+    "cfs_closure_set_params": (_i, [_vp, _vp]),
+    "cfs_closure_set_scales": (_i, [_vp, _vp, _vp]),
+    "cfs_closure_get_scales": (_i, [_vp, _vp, _vp]),
+    "cfs_closure_set_linear": (_i, [_vp, _vp]),
+    "cfs_closure_forward": (_i, [_vp, _vp, _vp, _i, _vp]),
+    "cfs_closure_gradients": (_i, [_vp, _vp, _vp, _i, _vp, _vp, _vp]),
+    "cfs_closure_train_step": (_i, [_vp, _vp, _vp, _vp, _i, ctypes.c_float, _vp, _vp]),
+    "cfs_closure_step": (_i, [_vp, _vp, _vp, _i, ctypes.c_double, _vp, _vp]),
     "cfs_encode_matrix": (_i, [_vp, _i, _i, _i, _i, _i, ctypes.c_double, _i, _i, ctypes.POINTER(_vp)]),
     "cfs_encoded_buffer": (_i, [_vp, _i, ctypes.POINTER(_vp), ctypes.POINTER(_sz)]),
     "cfs_encoded_write": (_i, [_vp, _cp]),

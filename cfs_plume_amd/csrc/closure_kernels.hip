@@ -1,0 +1,351 @@
+// closure_kernels.hip -- the reduced-advection step and its learned closure.
+//
+// Exact reduced advection (the quantity the reference's cubature step
+// reducedAdvectCompressionFriendly, SUBSPACE_FLUID_3D_COMPRESSED_EIGEN.cpp:
+// 1781-1854, approximates with key cells): q' = U^T A(U q), where A is the
+// first-order semi-Lagrangian self-advection of the full-grid velocity field
+// (VECTOR3_FIELD_3D::advect, VECTOR3_FIELD_3D.cpp:2001-2077: backtrace by
+// dt * v, clamp to [1.5, res - 2.5], trilinear interpolation).
+//
+// Learned closure: an MLP r -> H -> H -> H -> r (GELU after each hidden
+// layer) predicting the advection increment A(q) - q in scaled units.
+// Weights are BF16 with FP32 master copies for training; the batched forward
+// runs on MFMA (v_mfma_f32_16x16x16_bf16) with each activation split into a
+// BF16 hi + lo pair and FP32 accumulation; the backward pass is FP32 on the
+// vector unit; the optimiser is Adam.  The fused inference step evaluates the
+// same network with the BF16 weights and FP32 arithmetic for a few samples
+// per workgroup.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "mfma_idct.h"
+
+namespace cfs {
+
+// ---------------------------------------------------------------------------
+// semi-Lagrangian self-advection of a full-grid interleaved vector field
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) advect_kernel(const float* __restrict__ vel, float* __restrict__ out, int X,
+                                                     int Y, int Z, float dt0) {
+  const int n = X * Y * Z;
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n) return;
+  const int x = idx % X, y = (idx / X) % Y, z = idx / (X * Y);
+  float xt = (float)x - dt0 * vel[3 * idx];
+  float yt = (float)y - dt0 * vel[3 * idx + 1];
+  float zt = (float)z - dt0 * vel[3 * idx + 2];
+  xt = fminf(fmaxf(xt, 1.5f), (float)X - 2.5f);
+  yt = fminf(fmaxf(yt, 1.5f), (float)Y - 2.5f);
+  zt = fminf(fmaxf(zt, 1.5f), (float)Z - 2.5f);
+  const int x0 = (int)xt, y0 = (int)yt, z0 = (int)zt;
+  const float s1 = xt - (float)x0, s0 = 1.0f - s1;
+  const float t1 = yt - (float)y0, t0 = 1.0f - t1;
+  const float u1 = zt - (float)z0, u0 = 1.0f - u1;
+  const int slab = X * Y;
+  const int i000 = x0 + y0 * X + z0 * slab;
+  const int i010 = i000 + X, i100 = i000 + 1, i110 = i000 + X + 1;
+  const int i001 = i000 + slab, i011 = i010 + slab, i101 = i100 + slab, i111 = i110 + slab;
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    const float* f = vel + c;
+    out[3 * idx + c] = u0 * (s0 * (t0 * f[3 * i000] + t1 * f[3 * i010]) + s1 * (t0 * f[3 * i100] + t1 * f[3 * i110])) +
+                       u1 * (s0 * (t0 * f[3 * i001] + t1 * f[3 * i011]) + s1 * (t0 * f[3 * i101] + t1 * f[3 * i111]));
+  }
+}
+
+// q_out = alpha * q_out + forcing (forcing may be null)
+__global__ void axpb_kernel(double* __restrict__ q, int r, double alpha, const double* __restrict__ forcing) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < r) q[j] = alpha * q[j] + (forcing ? forcing[j] : 0.0);
+}
+
+// ---------------------------------------------------------------------------
+// MLP pieces
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+__device__ __forceinline__ float gelu_grad(float z) {
+  return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) + z * 0.39894228040143268f * expf(-0.5f * z * z);
+}
+
+// Y[n x M] = act(X[n x K] W + b) on MFMA.  X row-major FP32 (leading
+// dimension ldx >= K, K a multiple of 16); W packed BF16: element (k, m) at
+// Wp[((k / 4) * M + m) * 4 + k % 4], so a lane's four K values are one 8-byte
+// load; M a multiple of 64.  Workgroup: 4 waves, wave w computes the 16 x 16
+// tile of rows 16 blockIdx.x.., columns 64 blockIdx.y + 16 w...  A operand:
+// lane l holds row l % 16, K = 4 (l / 16) .. +3, split into BF16 hi + lo (two
+// products).  Z (pre-activation) is stored when non-null.
+__global__ void __launch_bounds__(256) mlp_fwd_kernel(const float* __restrict__ X, int ldx, int n,
+                                                      const uint16_t* __restrict__ Wp, const float* __restrict__ bias,
+                                                      int K, int M, float* __restrict__ Z, float* __restrict__ Y,
+                                                      int act) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int m0 = 16 * blockIdx.x, n0 = 64 * blockIdx.y + 16 * w;
+  const int ar = m0 + (lane & 15), ak = 4 * (lane >> 4);
+  const bool arow = ar < n;
+  f32x4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
+  for (int k0 = 0; k0 < K; k0 += 16) {
+    float4 a = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (arow) a = *reinterpret_cast<const float4*>(X + (size_t)ar * ldx + k0 + ak);
+    short h0, h1, h2, h3, l0, l1, l2, l3;
+    split_bf16(a.x, h0, l0);
+    split_bf16(a.y, h1, l1);
+    split_bf16(a.z, h2, l2);
+    split_bf16(a.w, h3, l3);
+    const bf16x4_t ah = {h0, h1, h2, h3}, al = {l0, l1, l2, l3};
+    const bf16x4_t b = *reinterpret_cast<const bf16x4_t*>(Wp + ((size_t)((k0 + ak) >> 2) * M + n0 + (lane & 15)) * 4);
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, b, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, b, acc, 0, 0, 0);
+  }
+  // D layout: row 4 (l / 16) + i, column l % 16
+  const int col = n0 + (lane & 15);
+  const float bc = bias[col];
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const int row = m0 + 4 * (lane >> 4) + i;
+    if (row < n) {
+      const float z = acc[i] + bc;
+      if (Z) Z[(size_t)row * M + col] = z;
+      Y[(size_t)row * M + col] = act ? gelu(z) : z;
+    }
+  }
+}
+
+// C[m, n] = sum_k A(m, k) B(k, n), A(m, k) = A[m * sam + k * sak],
+// B(k, n) = B[k * sbk + n * sbn], C row-major with leading dimension ldc.
+// If G is non-null the result is multiplied by gelu'(G[m * ldc + n]).
+// 16 x 16 tiles staged through LDS, one output per thread (FP32).
+__global__ void __launch_bounds__(256) gemm_f32_kernel(const float* __restrict__ A, long sam, long sak,
+                                                       const float* __restrict__ B, long sbk, long sbn,
+                                                       float* __restrict__ C, int ldc, int M, int N, int K,
+                                                       const float* __restrict__ G) {
+  __shared__ float As[16][17], Bs[16][17];
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  const int m = 16 * blockIdx.y + ty, nn = 16 * blockIdx.x + tx;
+  float acc = 0.0f;
+  for (int k0 = 0; k0 < K; k0 += 16) {
+    const int am = 16 * blockIdx.y + ty, ak = k0 + tx;
+    As[ty][tx] = (am < M && ak < K) ? A[am * sam + ak * sak] : 0.0f;
+    const int bk = k0 + ty, bn = 16 * blockIdx.x + tx;
+    Bs[ty][tx] = (bk < K && bn < N) ? B[bk * sbk + bn * sbn] : 0.0f;
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < 16; kk++) acc += As[ty][kk] * Bs[kk][tx];
+    __syncthreads();
+  }
+  if (m < M && nn < N) {
+    if (G) acc *= gelu_grad(G[(size_t)m * ldc + nn]);
+    C[(size_t)m * ldc + nn] = acc;
+  }
+}
+
+// db[j] = sum over rows of dZ[row, j]
+__global__ void bias_grad_kernel(const float* __restrict__ dZ, int n, int M, float* __restrict__ db) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= M) return;
+  float s = 0.0f;
+  for (int row = 0; row < n; row++) s += dZ[(size_t)row * M + j];
+  db[j] = s;
+}
+
+// Mean-squared loss over the first r of the M output columns:
+// dY = 2 (Y - T) / (n r) on those columns, 0 on the padding; the sum of
+// squared errors is added to *loss (FP64).
+__global__ void __launch_bounds__(256) loss_grad_kernel(const float* __restrict__ Y, const float* __restrict__ T,
+                                                        int n, int M, int r, float* __restrict__ dY,
+                                                        double* __restrict__ loss) {
+  __shared__ double part[256];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double e2 = 0.0;
+  if (i < n * M) {
+    const int j = i % M;
+    float g = 0.0f;
+    if (j < r) {
+      const float d = Y[i] - T[i];
+      e2 = (double)d * (double)d;
+      g = 2.0f * d / ((float)n * (float)r);
+    }
+    dY[i] = g;
+  }
+  part[threadIdx.x] = e2;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) part[threadIdx.x] += part[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && loss) atomicAdd(loss, part[0]);
+}
+
+// rows idx[0..n) of src (leading dimension ld) into dst
+__global__ void gather_rows_kernel(const float* __restrict__ src, const int32_t* __restrict__ idx, int n, int ld,
+                                   float* __restrict__ dst) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n * ld) return;
+  const int row = i / ld, c = i % ld;
+  dst[i] = src[(size_t)idx[row] * ld + c];
+}
+
+__global__ void adam_kernel(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
+                            float* __restrict__ v, long n, float lr, float b1, float b2, float eps, float c1,
+                            float c2) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.0f - b1) * gi;
+  const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  p[i] -= lr * (mi * c1) / (sqrtf(vi * c2) + eps);
+}
+
+// BF16 copies of layer weights W [K x M] (FP32 master): packed for the MFMA
+// forward (Wp) and as FP32 values (Wq) for the backward pass.
+__global__ void pack_kernel(const float* __restrict__ W, int K, int M, uint16_t* __restrict__ Wp,
+                            float* __restrict__ Wq) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= K * M) return;
+  const int k = i / M, m = i % M;
+  const __bf16 h = (__bf16)W[i];
+  Wq[i] = (float)h;
+  Wp[((size_t)(k >> 2) * M + m) * 4 + (k & 3)] = (uint16_t)bf16_bits(h);
+}
+
+// Inference step for up to CL_NS samples per workgroup: q_out = alpha (q_in +
+// s_out * (MLP(x) + x L)) + forcing with x = s_in * q_in (L the optional FP32
+// linear skip; forcing per sample, [n][r], or null), FP64 q, FP32 network
+// arithmetic with
+// the BF16 weights.  Thread j computes output j of each layer for every
+// sample of the workgroup (reading its packed weight column with 8-byte
+// loads); activations live in LDS.
+constexpr int CL_NS = 4;
+constexpr int CL_H = 256;
+
+struct ClosureNet {
+  const uint16_t* Wp[4];
+  const float* b[4];
+  const float* lin;  // FP32 [rp][rp] linear skip added to the output (input-major), or null
+  const float* s_in;
+  const float* s_out;
+  int r, rp, H;
+};
+
+__global__ void __launch_bounds__(CL_H) closure_step_kernel(ClosureNet net, const double* __restrict__ q_in,
+                                                            double* __restrict__ q_out, int n, double alpha,
+                                                            const double* __restrict__ forcing) {
+  __shared__ float act[2][CL_NS][CL_H];
+  const int j = threadIdx.x;
+  const int s0 = CL_NS * blockIdx.x;
+  const int ns = min(CL_NS, n - s0);
+  for (int s = 0; s < CL_NS; s++)
+    act[0][s][j] = (s < ns && j < net.r) ? (float)q_in[(size_t)(s0 + s) * net.r + j] * net.s_in[j] : 0.0f;
+  __syncthreads();
+  // linear skip x L (FP32), kept in registers until the output
+  float lin[CL_NS];
+#pragma unroll
+  for (int s = 0; s < CL_NS; s++) lin[s] = 0.0f;
+  if (net.lin && j < net.rp) {
+    for (int k = 0; k < net.rp; k++) {
+      const float a = net.lin[(size_t)k * net.rp + j];
+#pragma unroll
+      for (int s = 0; s < CL_NS; s++) lin[s] += act[0][s][k] * a;
+    }
+  }
+  int cur = 0;
+  for (int l = 0; l < 4; l++) {
+    const int K = l == 0 ? net.rp : net.H;
+    const int M = l == 3 ? net.rp : net.H;
+    if (j < M) {
+      float acc[CL_NS];
+      const float bj = net.b[l][j];
+#pragma unroll
+      for (int s = 0; s < CL_NS; s++) acc[s] = bj;
+      const uint16_t* wc = net.Wp[l] + (size_t)j * 4;
+      for (int kq = 0; kq < K / 4; kq++) {
+        const uint2 wbits = *reinterpret_cast<const uint2*>(wc + (size_t)kq * M * 4);
+        const float w0 = __uint_as_float(wbits.x << 16), w1 = __uint_as_float(wbits.x & 0xffff0000u);
+        const float w2 = __uint_as_float(wbits.y << 16), w3 = __uint_as_float(wbits.y & 0xffff0000u);
+#pragma unroll
+        for (int s = 0; s < CL_NS; s++) {
+          const float4 x = *reinterpret_cast<const float4*>(&act[cur][s][4 * kq]);
+          acc[s] += x.x * w0 + x.y * w1 + x.z * w2 + x.w * w3;
+        }
+      }
+#pragma unroll
+      for (int s = 0; s < CL_NS; s++) act[cur ^ 1][s][j] = l < 3 ? gelu(acc[s]) : acc[s];
+    }
+    __syncthreads();
+    cur ^= 1;
+  }
+  if (j < net.r) {
+    for (int s = 0; s < ns; s++) {
+      const size_t o = (size_t)(s0 + s) * net.r + j;
+      const double qn = q_in[o] + (double)(act[cur][s][j] + lin[s]) * (double)net.s_out[j];
+      q_out[o] = alpha * qn + (forcing ? forcing[o] : 0.0);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// launchers (called from closure.cpp)
+// ---------------------------------------------------------------------------
+hipError_t launch_advect(const float* vel, float* out, int X, int Y, int Z, float dt0, hipStream_t s) {
+  const int n = X * Y * Z;
+  hipLaunchKernelGGL(advect_kernel, dim3((n + 255) / 256), dim3(256), 0, s, vel, out, X, Y, Z, dt0);
+  return hipGetLastError();
+}
+
+hipError_t launch_axpb(double* q, int r, double alpha, const double* forcing, hipStream_t s) {
+  hipLaunchKernelGGL(axpb_kernel, dim3((r + 255) / 256), dim3(256), 0, s, q, r, alpha, forcing);
+  return hipGetLastError();
+}
+
+hipError_t launch_mlp_fwd(const float* X, int ldx, int n, const uint16_t* Wp, const float* bias, int K, int M,
+                          float* Z, float* Y, int act, hipStream_t s) {
+  hipLaunchKernelGGL(mlp_fwd_kernel, dim3((n + 15) / 16, M / 64), dim3(256), 0, s, X, ldx, n, Wp, bias, K, M, Z, Y,
+                     act);
+  return hipGetLastError();
+}
+
+hipError_t launch_gemm_f32(const float* A, long sam, long sak, const float* B, long sbk, long sbn, float* C, int ldc,
+                           int M, int N, int K, const float* G, hipStream_t s) {
+  hipLaunchKernelGGL(gemm_f32_kernel, dim3((N + 15) / 16, (M + 15) / 16), dim3(256), 0, s, A, sam, sak, B, sbk, sbn,
+                     C, ldc, M, N, K, G);
+  return hipGetLastError();
+}
+
+hipError_t launch_bias_grad(const float* dZ, int n, int M, float* db, hipStream_t s) {
+  hipLaunchKernelGGL(bias_grad_kernel, dim3((M + 255) / 256), dim3(256), 0, s, dZ, n, M, db);
+  return hipGetLastError();
+}
+
+hipError_t launch_loss_grad(const float* Y, const float* T, int n, int M, int r, float* dY, double* loss,
+                            hipStream_t s) {
+  hipLaunchKernelGGL(loss_grad_kernel, dim3((n * M + 255) / 256), dim3(256), 0, s, Y, T, n, M, r, dY, loss);
+  return hipGetLastError();
+}
+
+hipError_t launch_gather_rows(const float* src, const int32_t* idx, int n, int ld, float* dst, hipStream_t s) {
+  hipLaunchKernelGGL(gather_rows_kernel, dim3((n * ld + 255) / 256), dim3(256), 0, s, src, idx, n, ld, dst);
+  return hipGetLastError();
+}
+
+hipError_t launch_adam(float* p, const float* g, float* m, float* v, long n, float lr, float b1, float b2, float eps,
+                       float c1, float c2, hipStream_t s) {
+  hipLaunchKernelGGL(adam_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, g, m, v, n, lr, b1, b2, eps,
+                     c1, c2);
+  return hipGetLastError();
+}
+
+hipError_t launch_pack(const float* W, int K, int M, uint16_t* Wp, float* Wq, hipStream_t s) {
+  hipLaunchKernelGGL(pack_kernel, dim3((K * M + 255) / 256), dim3(256), 0, s, W, K, M, Wp, Wq);
+  return hipGetLastError();
+}
+
+hipError_t launch_closure_step(const ClosureNet& net, const double* q_in, double* q_out, int n, double alpha,
+                               const double* forcing, hipStream_t s) {
+  hipLaunchKernelGGL(closure_step_kernel, dim3((n + CL_NS - 1) / CL_NS), dim3(CL_H), 0, s, net, q_in, q_out, n, alpha,
+                     forcing);
+  return hipGetLastError();
+}
+
+}  // namespace cfs

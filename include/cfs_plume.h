@@ -139,6 +139,60 @@ int cfs_batch_graph_capture(cfs_batch* batch, void* stream);
 int cfs_batch_graph_launch(cfs_batch* batch, void* stream);
 int cfs_batch_destroy(cfs_batch* batch);
 
+/* ---- reduced advection and its learned closure ------------------------------
+ * The reference advects in the subspace with cubature key cells
+ * (reducedAdvectCompressionFriendly, SUBSPACE_FLUID_3D_COMPRESSED_EIGEN.cpp:
+ * 1781-1854, called from stepPlume :481): q' = sum_i A_i (sum_corners w U q).
+ * cfs_reduced_advect evaluates the quantity that cubature approximates on
+ * every cell: q_out = alpha * U^T A(U q_in) + forcing, with A the
+ * first-order semi-Lagrangian self-advection VECTOR3_FIELD_3D::advect
+ * (VECTOR3_FIELD_3D.cpp:2001-2077) over the full grid with backtrace dt0
+ * (cells per unit velocity).  field_a / field_b: FP32 full-grid scratch
+ * fields; forcing: FP64[r] or NULL.  It is the fallback and the judge of the
+ * learned closure. */
+int cfs_reduced_advect(cfs_basis* U, const double* q_in, double* q_out, float* field_a, float* field_b, float dt0,
+                       double alpha, const double* forcing, void* stream);
+/* VECTOR3_FIELD_3D::advect of a full-grid field by itself (dims >= 4). */
+int cfs_advect_field(const float* vel, float* out, int X, int Y, int Z, float dt0, void* stream);
+
+/* Learned closure (no counterpart in the reference): an MLP r -> 256 -> 256 ->
+ * 256 -> r with GELU, BF16 weights (FP32 master copies) and FP32
+ * accumulation, predicting the scaled advection increment
+ * (A(q) - q) / s_out from s_in * q.  Parameters are one FP32 array: for each
+ * layer l = 0..3, W_l (K_l x M_l, row-major, input-major) then b_l (M_l),
+ * K_0 = M_3 = r_padded (r rounded up to 64; padding rows/columns are zero),
+ * other widths 256.  Network inputs/targets (forward, gradients, training)
+ * are device FP32 arrays of n rows of r_padded (padding columns zero). */
+typedef struct cfs_closure cfs_closure;
+int cfs_closure_create(int r, int hidden, int max_batch, uint64_t seed, cfs_closure** out);
+int cfs_closure_destroy(cfs_closure* c);
+int cfs_closure_get_dims(const cfs_closure* c, int* r, int* r_padded, int* hidden, int64_t* n_params);
+int cfs_closure_get_params(const cfs_closure* c, float* host_params);
+/* Sets the parameters (host FP32) and resets the optimiser state. */
+int cfs_closure_set_params(cfs_closure* c, const float* host_params);
+int cfs_closure_set_scales(cfs_closure* c, const float* s_in, const float* s_out);
+int cfs_closure_get_scales(const cfs_closure* c, float* s_in, float* s_out);
+/* Optional FP32 linear skip L (host r x r, input-major: out_j += sum_k x_k L[k][j])
+ * added to the network output by cfs_closure_step; NULL removes it.  The
+ * network is then trained on the residual t - x L. */
+int cfs_closure_set_linear(cfs_closure* c, const float* L);
+/* Batched forward on MFMA (activations split BF16 hi + lo): y = MLP(x). */
+int cfs_closure_forward(cfs_closure* c, const float* x, float* y, int n, void* stream);
+/* Gradients (device FP32 array laid out like the parameters) of the
+ * mean-squared error over the first r outputs, and the sum of squared errors
+ * (device FP64), without an update. */
+int cfs_closure_gradients(cfs_closure* c, const float* x, const float* t, int n, float* grads, double* loss,
+                          void* stream);
+/* One Adam step (beta 0.9 / 0.999, eps 1e-8) on rows `rows[0..n)` of the
+ * device training arrays x, t; the batch's sum of squared errors goes to
+ * *loss_sum (device FP64, may be NULL). */
+int cfs_closure_train_step(cfs_closure* c, const float* x, const float* t, const int32_t* rows, int n, float lr,
+                           double* loss_sum, void* stream);
+/* Closure step for n samples: q_out = alpha * (q_in + s_out * (MLP(x) + x L))
+ * + forcing with x = s_in * q_in; q FP64 [n][r], forcing FP64 [n][r] or NULL. */
+int cfs_closure_step(cfs_closure* c, const double* q_in, double* q_out, int n, double alpha, const double* forcing,
+                     void* stream);
+
 /* ---- encoder (host) ---------------------------------------------------------
  * CompressAndWriteMatrixComponents (COMPRESSION.cpp:2271-2359) with the SVD
  * transform off, as on the live path (:2312).  U is FP64 column-major:
