@@ -153,14 +153,19 @@ def run_closure(args):
     t0 = time.perf_counter()
     b_train = plume.forcing(runs[:n_train])
     traj = plume.rollout_exact(b_train, n_train, steps)
-    t_data = time.perf_counter() - t0
     q, d = CL.increments(traj, b_train.numpy(), plume.alpha)
-    s_in, s_out = CL.fit_scales(q, d)
+    if args.closure_noise > 0:
+        # the exact step also at perturbed states around the trajectories, so
+        # that the closure learns the map in a neighbourhood of them
+        rng = np.random.default_rng(11)
+        qp = q + args.closure_noise * rng.normal(size=q.shape) * np.linalg.norm(q, axis=1, keepdims=True) / np.sqrt(r)
+        dp = plume.exact_increments(qp)
+        q, d = np.concatenate([q, qp]), np.concatenate([d, dp])
+    t_data = time.perf_counter() - t0
     cl = C.Closure(r, max_batch=1024, seed=5)
-    cl.set_scales(s_in, s_out)
     t0 = time.perf_counter()
-    losses = cl.train(q * s_in, d / s_out, epochs=args.closure_epochs, batch=512, lr=2e-3, lr_final=1e-5, seed=7,
-                      log=lambda ep, m: log(f"[bench] closure epoch {ep}: mse {m:.3e}") if ep % 20 == 0 else None)
+    losses = cl.fit(q, d, epochs=args.closure_epochs, batch=512, lr=2e-3, lr_final=1e-5, seed=7,
+                    log=lambda ep, m: log(f"[bench] closure epoch {ep}: mse {m:.3e}") if ep % 20 == 0 else None)
     t_train = time.perf_counter() - t0
     log(f"[bench] closure: {n_train} runs x {steps} steps of exact data in {t_data:.1f}s, "
         f"{args.closure_epochs} epochs in {t_train:.1f}s, final mse {losses[-1]:.3e} (first {losses[0]:.3e})")
@@ -210,6 +215,7 @@ def run_closure(args):
             "drift_without_advection": round(d_pl, 6),
             "checked_every_10": {"drift_at_100": round(d_chk, 6), "fallbacks": falls},
             "train": {"samples": int(q.shape[0]), "epochs": args.closure_epochs, "batch": 512,
+                      "perturbed_states": args.closure_noise, "linear_skip": "least squares, fp32",
                       "mse_first": losses[0], "mse_last": losses[-1], "data_s": round(t_data, 2),
                       "train_s": round(t_train, 2)},
         },
@@ -252,6 +258,8 @@ def main():
     ap.add_argument("--workload", default=DEFAULT, choices=sorted(WORKLOADS) + [BATCH, CLOSURE])
     ap.add_argument("--closure-runs", type=int, default=200, help="training runs of the closure workload")
     ap.add_argument("--closure-epochs", type=int, default=200)
+    ap.add_argument("--closure-noise", type=float, default=0.1,
+                    help="relative size of the perturbed states added to the closure's training data")
     ap.add_argument("--cpu-sample-cols", type=int, default=200,
                     help="columns of the oracle CPU step to time (the full step when >= r)")
     ap.add_argument("--no-cpu", action="store_true")

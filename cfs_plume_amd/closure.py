@@ -289,6 +289,17 @@ class ReducedPlume:
                                self.dt0, self.alpha, (b, i), stream)
         return traj.numpy().reshape(runs, steps + 1, r)
 
+    def exact_increments(self, states: np.ndarray, stream: Optional[Stream] = None) -> np.ndarray:
+        """Adv(q) - q for every row of `states` ([N][r], host FP64), with the
+        exact reduced advection on the device."""
+        r = self.U.r
+        n = states.shape[0]
+        qin = DeviceArray.from_numpy(np.ascontiguousarray(states, np.float64))
+        qout = DeviceArray((n, r), np.float64)
+        for i in range(n):
+            reduced_advect(self.U, (qin, i), (qout, i), self.fa, self.fb, self.dt0, 1.0, None, stream)
+        return qout.numpy() - states
+
     def rollout_closure(self, closure: Closure, b: DeviceArray, runs: int, steps: int,
                         stream: Optional[Stream] = None) -> np.ndarray:
         """All runs advanced together by the closure step (n = runs per call)."""

@@ -229,54 +229,76 @@ struct ClosureNet {
   int r, rp, H;
 };
 
-__global__ void __launch_bounds__(CL_H) closure_step_kernel(ClosureNet net, const double* __restrict__ q_in,
-                                                            double* __restrict__ q_out, int n, double alpha,
-                                                            const double* __restrict__ forcing) {
-  __shared__ float act[2][CL_NS][CL_H];
-  const int j = threadIdx.x;
-  const int s0 = CL_NS * blockIdx.x;
-  const int ns = min(CL_NS, n - s0);
-  for (int s = 0; s < CL_NS; s++)
-    act[0][s][j] = (s < ns && j < net.r) ? (float)q_in[(size_t)(s0 + s) * net.r + j] * net.s_in[j] : 0.0f;
+// NS samples per workgroup; KS thread groups of CL_H split each dot product
+// over K (partial sums combined through LDS), so that KS * CL_H threads keep
+// weight loads in flight.
+template <int NS, int KS>
+__global__ void __launch_bounds__(CL_H * KS) closure_step_kernel(ClosureNet net, const double* __restrict__ q_in,
+                                                                 double* __restrict__ q_out, int n, double alpha,
+                                                                 const double* __restrict__ forcing) {
+  __shared__ float act[2][NS][CL_H];
+  __shared__ float part[KS > 1 ? KS - 1 : 1][NS][CL_H];
+  const int j = threadIdx.x % CL_H, ks = threadIdx.x / CL_H;
+  const int s0 = NS * blockIdx.x;
+  const int ns = min(NS, n - s0);
+  if (ks == 0)
+    for (int s = 0; s < NS; s++)
+      act[0][s][j] = (s < ns && j < net.r) ? (float)q_in[(size_t)(s0 + s) * net.r + j] * net.s_in[j] : 0.0f;
   __syncthreads();
   // linear skip x L (FP32), kept in registers until the output
-  float lin[CL_NS];
+  float lin[NS];
 #pragma unroll
-  for (int s = 0; s < CL_NS; s++) lin[s] = 0.0f;
-  if (net.lin && j < net.rp) {
+  for (int s = 0; s < NS; s++) lin[s] = 0.0f;
+  if (ks == 0 && net.lin && j < net.rp) {
     for (int k = 0; k < net.rp; k++) {
       const float a = net.lin[(size_t)k * net.rp + j];
 #pragma unroll
-      for (int s = 0; s < CL_NS; s++) lin[s] += act[0][s][k] * a;
+      for (int s = 0; s < NS; s++) lin[s] += act[0][s][k] * a;
     }
   }
   int cur = 0;
   for (int l = 0; l < 4; l++) {
     const int K = l == 0 ? net.rp : net.H;
     const int M = l == 3 ? net.rp : net.H;
-    if (j < M) {
-      float acc[CL_NS];
-      const float bj = net.b[l][j];
+    float acc[NS];
 #pragma unroll
-      for (int s = 0; s < CL_NS; s++) acc[s] = bj;
+    for (int s = 0; s < NS; s++) acc[s] = 0.0f;
+    if (j < M) {
+      // this group's quarter (KS = 4) of the K range, in packed groups of 4
+      const int q0 = ks * (K / 4) / KS, q1 = (ks + 1) * (K / 4) / KS;
       const uint16_t* wc = net.Wp[l] + (size_t)j * 4;
-      for (int kq = 0; kq < K / 4; kq++) {
+      for (int kq = q0; kq < q1; kq++) {
         const uint2 wbits = *reinterpret_cast<const uint2*>(wc + (size_t)kq * M * 4);
         const float w0 = __uint_as_float(wbits.x << 16), w1 = __uint_as_float(wbits.x & 0xffff0000u);
         const float w2 = __uint_as_float(wbits.y << 16), w3 = __uint_as_float(wbits.y & 0xffff0000u);
 #pragma unroll
-        for (int s = 0; s < CL_NS; s++) {
+        for (int s = 0; s < NS; s++) {
           const float4 x = *reinterpret_cast<const float4*>(&act[cur][s][4 * kq]);
           acc[s] += x.x * w0 + x.y * w1 + x.z * w2 + x.w * w3;
         }
       }
+    }
+    if (KS > 1) {
+      if (ks > 0)
 #pragma unroll
-      for (int s = 0; s < CL_NS; s++) act[cur ^ 1][s][j] = l < 3 ? gelu(acc[s]) : acc[s];
+        for (int s = 0; s < NS; s++) part[ks - 1][s][j] = acc[s];
+      __syncthreads();
+    }
+    if (ks == 0 && j < M) {
+      const float bj = net.b[l][j];
+#pragma unroll
+      for (int s = 0; s < NS; s++) {
+        float z = acc[s];
+#pragma unroll
+        for (int g = 1; g < KS; g++) z += part[g - 1][s][j];
+        z += bj;
+        act[cur ^ 1][s][j] = l < 3 ? gelu(z) : z;
+      }
     }
     __syncthreads();
     cur ^= 1;
   }
-  if (j < net.r) {
+  if (ks == 0 && j < net.r) {
     for (int s = 0; s < ns; s++) {
       const size_t o = (size_t)(s0 + s) * net.r + j;
       const double qn = q_in[o] + (double)(act[cur][s][j] + lin[s]) * (double)net.s_out[j];
@@ -343,8 +365,13 @@ hipError_t launch_pack(const float* W, int K, int M, uint16_t* Wp, float* Wq, hi
 
 hipError_t launch_closure_step(const ClosureNet& net, const double* q_in, double* q_out, int n, double alpha,
                                const double* forcing, hipStream_t s) {
-  hipLaunchKernelGGL(closure_step_kernel, dim3((n + CL_NS - 1) / CL_NS), dim3(CL_H), 0, s, net, q_in, q_out, n, alpha,
-                     forcing);
+  // one sample (the closure step of one run): one workgroup, the K range split
+  // four ways; batches: four samples per workgroup
+  if (n == 1)
+    hipLaunchKernelGGL((closure_step_kernel<1, 4>), dim3(1), dim3(CL_H * 4), 0, s, net, q_in, q_out, n, alpha, forcing);
+  else
+    hipLaunchKernelGGL((closure_step_kernel<CL_NS, 1>), dim3((n + CL_NS - 1) / CL_NS), dim3(CL_H), 0, s, net, q_in,
+                       q_out, n, alpha, forcing);
   return hipGetLastError();
 }
 

@@ -182,8 +182,12 @@ def test_closure_tracks_held_out_runs(plume):
     b_train = plume.forcing(train)
     traj = plume.rollout_exact(b_train, len(train), steps)
     q, d = CL.increments(traj, b_train.numpy(), plume.alpha)
+    # exact increments also at states perturbed by 10 % around the trajectories
+    rng = np.random.default_rng(23)
+    qp = q + 0.1 * rng.normal(size=q.shape) * np.linalg.norm(q, axis=1, keepdims=True) / np.sqrt(q.shape[1])
+    q, d = np.concatenate([q, qp]), np.concatenate([d, plume.exact_increments(qp)])
     cl = Closure(plume.U.r, max_batch=256, seed=17)
-    print("[case] plume closure training 1440 samples", flush=True)
+    print("[case] plume closure training 2880 samples", flush=True)
     losses = cl.fit(q, d, epochs=200, batch=128, lr=2e-3, lr_final=1e-5, seed=19)
     assert losses[-1] < 0.2 * losses[0]
     b_held = plume.forcing(held)
