@@ -139,13 +139,25 @@ __global__ void __launch_bounds__(256) gemm_f32_kernel(const float* __restrict__
   }
 }
 
-// db[j] = sum over rows of dZ[row, j]
-__global__ void bias_grad_kernel(const float* __restrict__ dZ, int n, int M, float* __restrict__ db) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= M) return;
+// db[j] = sum over rows of dZ[row, j]: a workgroup takes 16 columns, its 16
+// row groups sum every 16th row, and the partial sums meet in LDS (fixed
+// order, so the result does not depend on scheduling).
+__global__ void __launch_bounds__(256) bias_grad_kernel(const float* __restrict__ dZ, int n, int M,
+                                                        float* __restrict__ db) {
+  __shared__ float part[16][17];
+  const int c = threadIdx.x & 15, g = threadIdx.x >> 4;
+  const int j = 16 * blockIdx.x + c;
   float s = 0.0f;
-  for (int row = 0; row < n; row++) s += dZ[(size_t)row * M + j];
-  db[j] = s;
+  if (j < M)
+    for (int row = g; row < n; row += 16) s += dZ[(size_t)row * M + j];
+  part[g][c] = s;
+  __syncthreads();
+  if (g == 0 && j < M) {
+    float t = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 16; k++) t += part[k][c];
+    db[j] = t;
+  }
 }
 
 // Mean-squared loss over the first r of the M output columns:
@@ -336,7 +348,7 @@ hipError_t launch_gemm_f32(const float* A, long sam, long sak, const float* B, l
 }
 
 hipError_t launch_bias_grad(const float* dZ, int n, int M, float* db, hipStream_t s) {
-  hipLaunchKernelGGL(bias_grad_kernel, dim3((M + 255) / 256), dim3(256), 0, s, dZ, n, M, db);
+  hipLaunchKernelGGL(bias_grad_kernel, dim3((M + 15) / 16), dim3(256), 0, s, dZ, n, M, db);
   return hipGetLastError();
 }
 
