@@ -228,8 +228,14 @@ spatial_kernel(DevBasis B, DecodeArgs a, const float* __restrict__ g_damp, const
   int2* prm = reinterpret_cast<int2*>(cbase + (size_t)SP_G * BC * 4);   // [SP_G + 1] {row byte offset, bits(1/s)}
   float* tile = reinterpret_cast<float*>(lds + spatial_shared_lds_bytes(nk) + 3 * spatial_comp_lds_bytes());
 
-  const long total = (long)((a.ncol + SP_G - 1) / SP_G) * B.nB;
-  long item = blockIdx.x;
+  // Each workgroup takes a contiguous range of items: neighbouring blocks
+  // along x, whose 96-byte rows share 128-byte lines of the output, are
+  // written one after the other by the same workgroup (same L2).
+  const long all = (long)((a.ncol + SP_G - 1) / SP_G) * B.nB;
+  const long per = (all + gridDim.x - 1) / gridDim.x;
+  long item = (long)blockIdx.x * per;
+  const long total = min(all, item + per);
+  if (item >= total) return;  // workgroup-uniform: no items (no barrier reached)
   SpatialItem cur;
   spatial_load(B, a, k, item, cur);
   StreamParser<SP_NT> parser = spatial_parser(B, k, cur);
@@ -244,7 +250,7 @@ spatial_kernel(DevBasis B, DecodeArgs a, const float* __restrict__ g_damp, const
   const size_t col_stride = (size_t)3 * B.X * B.Y * B.Z;
   bool complete = true;
 
-  for (; item < total; item += gridDim.x) {
+  for (; item < total; item++) {
     const int b = cur.b, c0 = cur.c0, G = cur.G;
     const size_t kb = (size_t)k * B.nB + b;
     const size_t si0 = kb * B.r + c0;
@@ -257,7 +263,7 @@ spatial_kernel(DevBasis B, DecodeArgs a, const float* __restrict__ g_damp, const
       for (int i = lane; i < SP_G * BC / 4; i += WAVE) x4[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
     __syncthreads();  // tables (first item) and this wave's buffers
-    const long nitem = item + gridDim.x;
+    const long nitem = item + 1;
     SpatialItem nx = cur;
     if (nitem < total) spatial_load(B, a, k, nitem, nx);
 
