@@ -232,6 +232,16 @@ def traffic_from_profiles(workload: str):
     return float(m.group(1)) if m else None
 
 
+def mfma_mops_from_profiles(workload: str):
+    """SQ_INSTS_VALU_MFMA_MOPS_BF16 per step launch (units of 512 FLOP) from the
+    committed PMC summary, if any."""
+    path = os.path.join(ROOT, "profiles", f"step_kernel_{workload}_pmc.md")
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"SQ_INSTS_VALU_MFMA_MOPS_BF16\s*\|\s*([0-9,]+)", open(path).read())
+    return float(m.group(1).replace(",", "")) if m else None
+
+
 def cpu_baseline(bufs, v, q, r, sample_cols):
     """The FP64 oracle (single thread) on columns [0, sample_cols); a partial
     sample is extrapolated linearly in the column count to the full step."""
@@ -388,6 +398,13 @@ def main():
             "traffic": traffic_from_profiles(args.workload),
         },
     }
+    mops = mfma_mops_from_profiles(args.workload)
+    if mops is not None and args.mode == "step":
+        # the inverse DCT is the only MFMA work (BF16 hi + lo products)
+        tflops = mops * 512 / (kernel_ms * 1e-3) / 1e12
+        result["mfma"] = {"flop_per_launch": mops * 512, "achieved_tflops": round(tflops, 2),
+                          "peak_tflops": 2500.0, "frac": round(tflops / 2500.0, 5),
+                          "counter": "SQ_INSTS_VALU_MFMA_MOPS_BF16 x 512 from the committed PMC profile"}
 
     # Spatial decode of every column (DecodeVectorField for c = 0..r-1): the
     # inverse DCTs run on MFMA (BF16 hi + lo products, FP32 accumulation).
@@ -423,6 +440,22 @@ def main():
                        f"FP64 oracle project+unproject of columns 0..{ns - 1} of {r} "
                        f"({t_sample:.1f} s), extrapolated linearly in columns to {t_full:.1f} s/step"),
         }
+        # the same FP64 step split over the host's cores (identical results);
+        # the reference is single-threaded, so this is a reported extra
+        threads = min(args.threads, os.cpu_count() or 1)
+        from oracle import oracle as O
+        ob = O.Basis(buffers=bufs)
+        t0 = time.perf_counter()
+        q_mt = ob.project_mt(v, threads)
+        f_mt = ob.unproject_mt(q, np.zeros_like(v), threads)
+        t_mt = time.perf_counter() - t0
+        ob.close()
+        same = ns == r and np.array_equal(q_mt, q_s) and np.array_equal(f_mt, f_s)
+        result["cpu_baseline_all_cores"] = {
+            "value": round(1.0 / t_mt, 5), "unit": "steps/s", "cores": threads, "kind": "port",
+            "sample": f"the same FP64 step on {threads} threads ({t_mt:.2f} s), results identical: {bool(same)}",
+        }
+        log(f"[bench] CPU oracle step: 1 thread {t_full:.2f} s, {threads} threads {t_mt:.2f} s")
         # parity of the timed GPU step on the sampled columns
         zq = DeviceArray.from_numpy(np.concatenate([q[:ns], np.zeros(r - ns)]))
         fs = DeviceArray.zeros((len(v),), np.float32)

@@ -19,12 +19,15 @@
 //   project / unproject : dense FP64 U^T v / U q on the spatially decoded basis
 //
 // Build: oracle/Makefile (g++ -O2 -ffp-contract=off, single-threaded).
+#include <algorithm>
 #include <cmath>
+#include <functional>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace {
@@ -682,6 +685,106 @@ int orc_unproject(void* h, const double* q, double* V, int col0, int col1) {
       }
     }
     for (int b = 0; b < nB; b++) dct3(acc.data() + (size_t)b * BC, false);
+    assimilate_unpadded(acc.data(), X, Y, Z, F.data());
+    for (int z = 0; z < Z; z++)
+      for (int y = 0; y < Y; y++)
+        for (int x = 0; x < X; x++)
+          V[3 * ((x + 1) + (size_t)GX * ((y + 1) + (size_t)GY * (z + 1))) + k] =
+              F[x + (size_t)X * (y + (size_t)Y * z)];
+  }
+  return ORC_OK;
+}
+
+// Multi-threaded forms of orc_project / orc_unproject for the CPU baseline
+// measured on all cores (the reference itself is single-threaded,
+// projects/include.mk:15).  Work is split so that every sum keeps the
+// single-threaded order: project over columns, unproject over blocks (each
+// block's accumulator sums the columns in order), so the results are
+// identical to orc_project / orc_unproject.
+static void run_threads(int n, int threads, const std::function<void(int, int)>& fn) {
+  threads = std::max(1, std::min(threads, n));
+  std::vector<std::thread> th;
+  for (int t = 0; t < threads; t++) {
+    const int a = (int)((long)n * t / threads), b = (int)((long)n * (t + 1) / threads);
+    th.emplace_back([&fn, a, b] { fn(a, b); });
+  }
+  for (auto& x : th) x.join();
+}
+
+int orc_project_mt(void* h, const double* V, double* q, int col0, int col1, int threads) {
+  Basis* B = (Basis*)h;
+  const Component& c0 = B->comp[0];
+  int X = c0.X, Y = c0.Y, Z = c0.Z, nB = c0.nB;
+  if (col0 < 0 || col1 > c0.r || col0 > col1) return ORC_ERR_RANGE;
+  size_t N = (size_t)X * Y * Z;
+  int GX = X + 2, GY = Y + 2;
+  std::vector<double> vh[3];
+  std::vector<double> F(N);
+  for (int k = 0; k < 3; k++) {
+    for (int z = 0; z < Z; z++)
+      for (int y = 0; y < Y; y++)
+        for (int x = 0; x < X; x++)
+          F[x + (size_t)X * (y + (size_t)Y * z)] =
+              V[3 * ((x + 1) + (size_t)GX * ((y + 1) + (size_t)GY * (z + 1))) + k];
+    vh[k].resize((size_t)nB * BC);
+    get_blocks(F.data(), X, Y, Z, true, vh[k].data());
+    run_threads(nB, threads, [&](int a, int b) {
+      for (int i = a; i < b; i++) dct3(vh[k].data() + (size_t)i * BC, true);
+    });
+  }
+  int status = ORC_OK;
+  run_threads(col1 - col0, threads, [&](int a, int b) {
+    double blk[BC];
+    for (int col = col0 + a; col < col0 + b; col++) {
+      double total = 0.0;
+      for (int k = 0; k < 3; k++) {
+        double sum = 0.0;
+        for (int bb = 0; bb < nB; bb++) {
+          if (decode_coeffs(B->comp[k], bb, col, blk)) {
+            status = ORC_ERR_FORMAT;
+            return;
+          }
+          const double* w = vh[k].data() + (size_t)bb * BC;
+          double d = 0.0;
+          for (int i = 0; i < BC; i++) d += blk[i] * w[i];
+          sum += d;
+        }
+        total += sum;
+      }
+      q[col - col0] = total;
+    }
+  });
+  return status;
+}
+
+int orc_unproject_mt(void* h, const double* q, double* V, int col0, int col1, int threads) {
+  Basis* B = (Basis*)h;
+  const Component& c0 = B->comp[0];
+  int X = c0.X, Y = c0.Y, Z = c0.Z, nB = c0.nB;
+  if (col0 < 0 || col1 > c0.r || col0 > col1) return ORC_ERR_RANGE;
+  size_t N = (size_t)X * Y * Z;
+  int GX = X + 2, GY = Y + 2;
+  std::vector<double> F(N);
+  int status = ORC_OK;
+  for (int k = 0; k < 3; k++) {
+    std::vector<double> acc((size_t)nB * BC, 0.0);
+    run_threads(nB, threads, [&](int a, int bend) {
+      double blk[BC];
+      for (int b = a; b < bend; b++) {
+        double* ac = acc.data() + (size_t)b * BC;
+        for (int col = col0; col < col1; col++) {
+          const double qc = q[col - col0];
+          if (decode_coeffs_table(B->comp[k], b, col, blk)) {
+            status = ORC_ERR_FORMAT;
+            return;
+          }
+          for (int i = 0; i < BC; i++)
+            if (blk[i] != 0.0) ac[i] += qc * blk[i];
+        }
+        dct3(ac, false);
+      }
+    });
+    if (status) return status;
     assimilate_unpadded(acc.data(), X, Y, Z, F.data());
     for (int z = 0; z < Z; z++)
       for (int y = 0; y < Y; y++)

@@ -58,6 +58,8 @@ def _load():
         "orc_decode_block_coeffs": (c_int, [c_void_p, c_int, c_int, c_int, c_int, _f64p]),
         "orc_decode_vector_field": (c_int, [c_void_p, c_int, _f64p]),
         "orc_project": (c_int, [c_void_p, _f64p, _f64p, c_int, c_int]),
+        "orc_project_mt": (c_int, [c_void_p, _f64p, _f64p, c_int, c_int, c_int]),
+        "orc_unproject_mt": (c_int, [c_void_p, _f64p, _f64p, c_int, c_int, c_int]),
         "orc_unproject": (c_int, [c_void_p, _f64p, _f64p, c_int, c_int]),
         "orc_get_submatrix": (c_int, [c_void_p, c_int, _f64p]),
         "orc_untransform_svd": (None, [_f64p, _f64p, ctypes.c_int64]),
@@ -288,6 +290,22 @@ class Basis:
         assert qq.size == col1 - col0
         V = np.array(V_full, np.float64, copy=True).reshape(-1)
         _check(lib().orc_unproject(self._h, qq, V, col0, col1), "PeeledCompressedUnprojectTransform")
+        return V
+
+    def project_mt(self, V_full, threads, col0=0, col1=None):
+        """project() split over `threads` threads (identical results)."""
+        col1 = self.r if col1 is None else col1
+        V = np.ascontiguousarray(V_full, np.float64).reshape(-1)
+        q = np.zeros(col1 - col0, np.float64)
+        _check(lib().orc_project_mt(self._h, V, q, col0, col1, threads), "project (threads)")
+        return q
+
+    def unproject_mt(self, q, V_full, threads, col0=0, col1=None):
+        """unproject() split over `threads` threads (identical results)."""
+        col1 = self.r if col1 is None else col1
+        qq = np.ascontiguousarray(q, np.float64)
+        V = np.array(V_full, np.float64, copy=True).reshape(-1)
+        _check(lib().orc_unproject_mt(self._h, qq, V, col0, col1, threads), "unproject (threads)")
         return V
 
     def get_submatrix(self, start_row):

@@ -380,3 +380,19 @@ def test_encoder_invariants(seed):
     d = O.damping_list(32)[int(4 * gamma)]
     # rint quantisation of the scaled, damped coefficients
     assert np.array_equal(ints, np.rint(F * s / d).astype(np.int64).astype(np.int32))
+
+
+def test_threaded_project_unproject_are_identical():
+    """The all-cores CPU baseline splits the oracle's work without changing any
+    summation order: results equal the single-threaded ones bit for bit."""
+    from cfs_plume_amd import encode_matrix, synth
+    dims = (14, 10, 9)
+    bufs = encode_matrix(synth.synthetic_basis(dims, 5, seed=2), dims).buffers()
+    ob = O.Basis(buffers=bufs)
+    rng = np.random.default_rng(4)
+    X, Y, Z = dims
+    v = rng.normal(size=3 * (X + 2) * (Y + 2) * (Z + 2))
+    q = rng.normal(size=5)
+    for threads in (1, 3, 8):
+        assert np.array_equal(ob.project_mt(v, threads), ob.project(v))
+        assert np.array_equal(ob.unproject_mt(q, np.zeros_like(v), threads), ob.unproject(q, np.zeros_like(v)))
