@@ -82,3 +82,21 @@ def test_synthetic_basis_is_orthonormal_and_matches_sampled_qr():
     B = synth.synthetic_basis(dims, 6, seed=2, direct_qr=True)
     assert np.max(np.abs(A @ A.T - np.eye(6))) < 1e-12
     assert np.max(np.abs(A - B)) < 1e-12
+
+
+def test_closure_and_advection_entry_points_reject_bad_arguments():
+    """Argument checks of the closure / reduced-advection ABI happen before any
+    device call, so they are testable without a GPU."""
+    import ctypes
+    lib = _lib.lib()
+    h = ctypes.c_void_p()
+    for r, hidden, batch in ((0, 256, 16), (300, 256, 16), (52, 128, 16), (52, 256, 0)):
+        assert lib.cfs_closure_create(r, hidden, batch, 1, ctypes.byref(h)) == -1, (r, hidden, batch)
+        assert not h.value
+    assert lib.cfs_closure_create(52, 256, 16, 1, None) == -1
+    assert lib.cfs_advect_field(None, None, 8, 8, 8, 1.0, None) == -1
+    buf = (ctypes.c_float * 3)()
+    assert lib.cfs_advect_field(buf, buf, 3, 8, 8, 1.0, None) == -1   # dims < 4
+    assert lib.cfs_reduced_advect(None, None, None, None, None, 1.0, 1.0, None, None) == -1
+    assert lib.cfs_closure_step(None, None, None, 1, 1.0, None, None) == -1
+    assert "null" in lib.cfs_last_error().decode()
