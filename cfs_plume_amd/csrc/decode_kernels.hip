@@ -161,17 +161,19 @@ decode_kernel(DevBasis B, DecodeArgs a, const float* __restrict__ g_damp, const 
 // as the (y, z) pass's B operand as it stands: K index 32 ks + 8 g + i maps
 // to row (w, v) = 32 ks + 4 g + i (i < 4) or 32 ks + 16 + 4 g + i - 4, and the
 // host orders the columns of K the same way.  Each product of a hi + lo pair
-// keeps hi*hi + hi*lo + lo*hi.  The three components meet in an LDS tile of
-// interleaved cells, written to the column as contiguous 96-byte rows.
+// keeps hi*hi + hi*lo + lo*hi.  Each wave writes its transformed pair over
+// the pair's (already read) coefficient buffers; the store phase gathers the
+// three components from the three waves' buffers and writes the column as
+// contiguous 96-byte rows of interleaved cells.
 // ---------------------------------------------------------------------------
 constexpr int SP_G = 4;    // columns per work item
 constexpr int SP_NT = 4;   // tokens per lane per chunk of the parser
 
-// LDS: damping rows | jinfo[512] | per component: X[SP_G][512] + prm[SP_G+1] | cell tile [2][512][3]
+// LDS: damping rows | jinfo[512] | per component: X[SP_G][512] + prm[SP_G+1]
 __host__ __device__ inline size_t spatial_shared_lds_bytes(int nk) { return ((size_t)nk * NDIAG * 4 + 15) / 16 * 16 + BC * 4; }
 __host__ __device__ inline size_t spatial_comp_lds_bytes() { return (size_t)SP_G * BC * 4 + (SP_G + 2) * 8; }
 __host__ inline size_t spatial_lds_bytes(int nk) {
-  return spatial_shared_lds_bytes(nk) + 3 * spatial_comp_lds_bytes() + (size_t)2 * BC * 3 * 4;
+  return spatial_shared_lds_bytes(nk) + 3 * spatial_comp_lds_bytes();
 }
 
 // One work item's column group and this wave's segment parameters (lane < G).
@@ -214,7 +216,7 @@ __device__ __forceinline__ StreamParser<SP_NT> spatial_parser(const DevBasis& B,
 
 // Persistent: workgroups stride over the work items; the next item's
 // parameters and first token chunks are loaded while this one transforms.
-__global__ void __launch_bounds__(3 * WAVE)
+__global__ void __launch_bounds__(3 * WAVE) __attribute__((amdgpu_waves_per_eu(3)))  // <= 168 registers
 spatial_kernel(DevBasis B, DecodeArgs a, const float* __restrict__ g_damp, const Tables* __restrict__ g_tab) {
   extern __shared__ __align__(16) char lds[];
   const int nk = B.nk;
@@ -226,7 +228,6 @@ spatial_kernel(DevBasis B, DecodeArgs a, const float* __restrict__ g_damp, const
   char* cbase = lds + spatial_shared_lds_bytes(nk) + (size_t)k * spatial_comp_lds_bytes();
   float* X = reinterpret_cast<float*>(cbase);                           // [SP_G][512]
   int2* prm = reinterpret_cast<int2*>(cbase + (size_t)SP_G * BC * 4);   // [SP_G + 1] {row byte offset, bits(1/s)}
-  float* tile = reinterpret_cast<float*>(lds + spatial_shared_lds_bytes(nk) + 3 * spatial_comp_lds_bytes());
 
   // Each workgroup takes a contiguous range of items: neighbouring blocks
   // along x, whose 96-byte rows share 128-byte lines of the output, are
@@ -324,19 +325,23 @@ spatial_kernel(DevBasis B, DecodeArgs a, const float* __restrict__ g_damp, const
       for (int t = 0; t < 4; t++) in[t] = *reinterpret_cast<const float4*>(xs + (16 * t + n) * BS);
       f32x4_t d2[4];
       idct_pair_mfma(in, mc, d2);
-      // lane column n = (block s0 + n / 8, x = n % 8), rows (z, y) = 16 t + 4 g + i;
-      // tile[(block, cell x + 8y + 64z)][k]
-      float* tp = tile + ((size_t)(n >> 3) * BC + (n & 7)) * 3 + k;
+      // lane column n = (block s0 + n / 8, x = n % 8), rows (z, y) = 16 t + 4 g + i,
+      // written over this wave's buffers of the pair: X[s0 + block][x + 8y + 64z]
+      // (the MFMAs above consumed every lane's loads of them)
+      float* tp = X + (size_t)(s0 + (n >> 3)) * BC + (n & 7);
 #pragma unroll
       for (int t = 0; t < 4; t++)
 #pragma unroll
-        for (int i = 0; i < 4; i++) tp[(16 * t + 4 * g + i) * BS * 3] = d2[t][i];
-      __syncthreads();  // the three components of the pair's cells are in the tile
+        for (int i = 0; i < 4; i++) tp[(16 * t + 4 * g + i) * BS] = d2[t][i];
+      __syncthreads();  // the three components of the pair's cells are in LDS
       // rows of 8 cells x 3 components = 24 contiguous floats per (block, z, y):
       // thread (y, f) = (tid / 24, tid % 24) writes float f of row (z, y) for every z
       {
         const int f24 = threadIdx.x % 24, y = threadIdx.x / 24;
-        const int gx = bx * BS + f24 / 3, gy = by * BS + y;
+        const int comp = f24 % 3, xx = f24 / 3;
+        const float* pl = reinterpret_cast<const float*>(lds + spatial_shared_lds_bytes(nk) +
+                                                         (size_t)comp * spatial_comp_lds_bytes()) + (size_t)s0 * BC;
+        const int gx = bx * BS + xx, gy = by * BS + y;
         const int zmax = min(BS, B.Z - bz * BS);
         if (gx < B.X && gy < B.Y) {
           const int cell0 = 3 * (gx + B.X * (gy + B.Y * bz * BS)) + f24 % 3;
@@ -345,15 +350,15 @@ spatial_kernel(DevBasis B, DecodeArgs a, const float* __restrict__ g_damp, const
           for (int sb = 0; sb < 2; sb++) {
             if (s0 + sb < G) {
               float* o = reinterpret_cast<float*>(a.out) + (size_t)(c0 + s0 + sb - a.col0) * col_stride + cell0;
-              const float* tr = tile + (sb * BC + y * BS) * 3 + f24;
+              const float* tr = pl + sb * BC + y * BS + xx;
 #pragma unroll
               for (int z = 0; z < BS; z++)
-                if (z < zmax) o[z * zstep] = tr[z * 64 * 3];
+                if (z < zmax) o[z * zstep] = tr[z * 64];
             }
           }
         }
       }
-      __syncthreads();  // the tile is free for the next pair
+      __syncthreads();  // the pair's buffers are free again
     }
     cur = nx;
   }
@@ -414,7 +419,10 @@ hipError_t launch_decode(const DevBasis& B, int mode, const DecodeArgs& a, const
     if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, spatial_kernel, 3 * WAVE, lds);
     if (e != hipSuccess) return e;
     if (per < 1) return hipErrorInvalidConfiguration;
-    const long grid = std::min<long>(items, (long)cus * per);
+    // two workgroups per CU: fewer items in flight per XCD between a block's
+    // partial output lines and its x-neighbour's completing them (measured
+    // 3 % faster than the full occupancy of 4)
+    const long grid = std::min<long>(items, (long)cus * std::min(per, 2));
     hipLaunchKernelGGL(spatial_kernel, dim3((unsigned)grid), dim3(3 * WAVE), lds, s, B, a, d_damp,
                        reinterpret_cast<const Tables*>(d_tab));
     return hipGetLastError();
