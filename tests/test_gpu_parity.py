@@ -125,7 +125,29 @@ def runs(tmp_path_factory, gpu):
     return _pair(H.basis_from_ints(dims, 6, runs_zz, seed=13), tmp_path_factory, "runs")
 
 
-CASES = ["smooth", "rough", "adversarial", "hazard", "runs"]
+def wide_zz(k, b, c):
+    """256 columns (the supported maximum): stretches of all-zero segments
+    (3 tokens each, so more than 64 segments close inside one 512-token
+    chunk), dense segments and runs, so every lane's four column slots and the
+    repeated segment-close pass are used."""
+    kind = (c // 80 + k + b) % 4
+    if kind == 0 or (kind == 1 and c % 3):
+        return np.zeros(512, np.int32)
+    if kind == 1:
+        return adversarial_zz(k, b, c % 7)
+    if kind == 2:
+        return runs_zz(k, b, c)
+    return hazard_zz(k, b, c)
+
+
+@pytest.fixture(scope="module")
+def wide(tmp_path_factory, gpu):
+    dims = (16, 8, 8)
+    print(f"[case] wide basis dims={dims} r=256", flush=True)
+    return _pair(H.basis_from_ints(dims, 256, wide_zz, seed=14), tmp_path_factory, "wide")
+
+
+CASES = ["smooth", "rough", "adversarial", "hazard", "runs", "wide"]
 
 
 def test_hazards_are_listed(hazard):

@@ -58,3 +58,14 @@ def ob_segment(path, b, c):
     tok = np.frombuffer(raw, np.int32, offset=40 + 24 * m)
     e = b + nB * c
     return tok[idx[e]: idx[e] + lens[e]]
+
+
+def test_more_columns_than_supported_are_rejected(tmp_path):
+    """The step keeps four column slots per lane (256 columns); a basis with
+    more columns is refused at load, before anything is uploaded."""
+    from cfs_plume_amd import CfsError
+    dims = (8, 8, 8)
+    bufs = H.basis_from_ints(dims, 257, lambda k, b, c: np.zeros(512, np.int32))
+    with pytest.raises(CfsError) as e:
+        CompressedBasis.from_files(H.write_files(bufs, str(tmp_path / "wide.")))
+    assert "CFS_ERR_FORMAT" in str(e.value) and "exceeds the supported maximum 256" in str(e.value), str(e.value)
