@@ -148,8 +148,8 @@ int cfs_batch_destroy(cfs_batch* batch);
  * first-order semi-Lagrangian self-advection VECTOR3_FIELD_3D::advect
  * (VECTOR3_FIELD_3D.cpp:2001-2077) over the full grid with backtrace dt0
  * (cells per unit velocity).  field_a / field_b: FP32 full-grid scratch
- * fields; forcing: FP64[r] or NULL.  It is the fallback and the judge of the
- * learned closure. */
+ * fields; forcing: FP64[r] or NULL.  It is the learned closure's fallback and
+ * the reference its accuracy is measured against. */
 int cfs_reduced_advect(cfs_basis* U, const double* q_in, double* q_out, float* field_a, float* field_b, float dt0,
                        double alpha, const double* forcing, void* stream);
 /* VECTOR3_FIELD_3D::advect of a full-grid field by itself (dims >= 4). */
