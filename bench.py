@@ -112,6 +112,22 @@ def run_batch(args):
     step_bytes = sum(b.step_bytes() for b in bases)
     achieved = step_bytes / (float(per.mean()) * 1e-3) / 1e9
     log(f"[bench] batch of {len(bases)}: {ms_step:.4f} ms/step, B_step={step_bytes / 1e9:.3f} GB -> {achieved:.0f} GB/s")
+    # the same work as one fused-step launch per basis, back to back on the stream
+    seq_reps = max(1, min(args.steps, 20))
+    for b, (v_, qo_, q_, f_) in zip(bases, arrs):
+        C.step(b, b, v_, qo_, q_, f_, stream)
+    e0, e1 = Event(), Event()
+    e0.record(stream)
+    for _ in range(seq_reps):
+        for b, (v_, qo_, q_, f_) in zip(bases, arrs):
+            C.step(b, b, v_, qo_, q_, f_, stream)
+    e1.record(stream)
+    stream.synchronize()
+    for b in bases:
+        b.raise_on_stream_errors()
+    seq_ms = e0.elapsed_ms(e1) / seq_reps
+    log(f"[bench] the same {len(bases)} bases one launch each: {seq_ms:.4f} ms/step "
+        f"(batched launch {seq_ms / ms_step:.2f}x faster)")
     result = {
         "metric": f"batched reduced steps/s ({len(bases)} bases, decode+project+unproject); % HBM peak",
         "value": round(1000.0 / ms_step, 2), "unit": "steps/s", "n_gpus": 1, "steps": args.steps,
@@ -122,6 +138,7 @@ def run_batch(args):
                         f"autotuned launch shape",
             "bases": [{"grid": list(g), "r": r, "percent": round(p, 5)} for g, r, p, _ in specs],
             "bytes_per_step": step_bytes, "launch": cfg, "parallelism": "single GPU",
+            "sequential_ms_per_step": round(seq_ms, 5),
         },
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None},
