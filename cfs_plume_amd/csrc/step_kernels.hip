@@ -12,8 +12,8 @@
 //
 // Precision: tokens and positions are exact integers; dequantisation is
 // int * (damp^gamma * (1/s)) in FP32 (the reference evaluates it in FP64,
-// COMPRESSION.cpp:1029-1056); per-block dots are FP32 and are accumulated
-// across blocks in FP64.
+// COMPRESSION.cpp:1029-1056); per-block dots are FP32, summed in FP32 over
+// the few blocks one wave handles and in FP64 across waves.
 #include <hip/hip_runtime.h>
 
 #include <type_traits>
@@ -38,14 +38,14 @@ constexpr int NO_END = 0x7fffffff;
 struct WaveLds {
   int4* prm;     // [r+1] {damp row byte offset, bits(fixed-point weight of q_c), bits(2^64/s), end offset};
                  // entry r is {0, 0, 0, NO_END} (tokens past the stream's end index it)
-  double* qacc;  // [r] this wave's FP64 project accumulator
+  float* qacc;   // [r] this wave's project accumulator (the dots of its items)
   float* vz;     // [512] DCT(v) block in zigzag order
   int* az;       // [512] unproject accumulator in zigzag order, fixed point
   float* pfx;    // [512] project prefix sums of the current chunk; DCT scratch
 };
 
 __host__ __device__ inline size_t step_wave_lds_bytes(int r) {
-  return (size_t)(r + 1) * 16 + 3 * (size_t)BC * 4 + ((size_t)r * 8 + 15) / 16 * 16;
+  return (size_t)(r + 1) * 16 + 3 * (size_t)BC * 4 + ((size_t)r * 4 + 15) / 16 * 16;
 }
 // Shared part of the workgroup's dynamic LDS: the damping rows (the tables
 // 4*(u+v+w) and zigzag by index are static LDS, 3*512 bytes).
@@ -63,7 +63,7 @@ __device__ __forceinline__ WaveLds carve_wave(char* base, int r) {
   w.vz = reinterpret_cast<float*>(p); p += BC * 4;
   w.az = reinterpret_cast<int*>(p); p += BC * 4;
   w.pfx = reinterpret_cast<float*>(p); p += BC * 4;
-  w.qacc = reinterpret_cast<double*>(p);
+  w.qacc = reinterpret_cast<float*>(p);
   return w;
 }
 
@@ -74,9 +74,10 @@ __device__ __forceinline__ WaveLds carve_wave(char* base, int r) {
 // partial sum of rounded terms stays below 2^31 and the rounding error is
 // below r units of 1/S per entry.
 //
-// Project keeps a running sum over the block stream of int * damp * DCT(v)
-// (FP32 within a chunk, FP64 across chunks).  The running sum at the last
-// token of segment c minus the sum at the last token of segment c-1, times
+// Project keeps, per chunk, the prefix sum of int * damp * DCT(v) over the
+// chunk's tokens (FP32, starting at zero).  The prefix at the last token of
+// segment c minus the prefix at the last token of segment c-1 (or, for the
+// first segment closed in a chunk, plus its sum over earlier chunks), times
 // 1/s_c, is segment c's dot product.  The encoder scales every segment to the
 // same peak (2^(nBits-1)-1), so the unscaled terms of neighbouring segments
 // have comparable magnitudes.
@@ -197,7 +198,7 @@ __device__ __forceinline__ void step_body(const DevBasis& B, const float* __rest
     zig[i] = g_tab->zig[i];
   }
   for (int i = lane; i < BC; i += WAVE) w.az[i] = 0;
-  for (int c = lane; c < r; c += WAVE) w.qacc[c] = 0.0;
+  for (int c = lane; c < r; c += WAVE) w.qacc[c] = 0.0f;
   if (lane == 0) w.prm[r] = make_int4(0, 0, 0, NO_END);
   // q is the same for every block: keep this lane's columns in registers
   float qf[QREG];
@@ -245,8 +246,7 @@ __device__ __forceinline__ void step_body(const DevBasis& B, const float* __rest
     MarkedStream<NT> parser(B.tokens + tbase, B.len_bits + (size_t)it.d2.w * (CH / 8), len);
     int next_lr = l0 < l1 ? (int)B.lr[l0] : NO_END;
     uint32_t li = l0;
-    double run = 0.0;       // project sum of the chunks before this one (wave-uniform)
-    double last_end = 0.0;  // running sum at the end of the last closed segment
+    float open_sum = 0.0f;  // project sum of segment cur's tokens in earlier chunks (wave-uniform)
     int cur = 0;            // first segment not yet closed
     // chunks alternate between the parser's two token buffers, so the
     // loads of chunk c+2 are not waited for until chunk c+2
@@ -329,27 +329,26 @@ __device__ __forceinline__ void step_body(const DevBasis& B, const float* __rest
         for (int q = 0; q < NT / 4; q++)
           pf[q] = make_float4(excl + pre[4 * q], excl + pre[4 * q + 1], excl + pre[4 * q + 2], excl + pre[4 * q + 3]);
         wave_lds_sync();
-        // segments cur, cur+1, ... whose last token lies in this chunk
+        // segments cur, cur+1, ... whose last token lies in this chunk: the
+        // dot of segment cur + j is the chunk-local prefix at its last token
+        // minus the one at the last token of segment cur + j - 1 (lane j - 1,
+        // or bv: the last segment closed before, or -open_sum for segment cur)
+        float bv = -open_sum;
         for (;;) {
           const int c = cur + lane;
           const bool closes = se.y <= base + CH;
           const int pi = closes ? se.y - 1 - base : 0;
           const float pv = w.pfx[pi];
-          const double q0 = closes ? w.qacc[c] : 0.0;
-          const double endv = run + (double)pv;
-          // running sum at the previous segment's end: lane-1's, or the last one closed
-          const int2 eb = make_int2(__double2loint(endv), __double2hiint(endv));
-          const double prevv = __hiloint2double((int)wave_shr1((uint32_t)eb.y, (uint32_t)__double2hiint(last_end)),
-                                                (int)wave_shr1((uint32_t)eb.x, (uint32_t)__double2loint(last_end)));
-          if (closes) w.qacc[c] = q0 + (endv - prevv) * (double)__int_as_float(se.x);
+          const float q0 = closes ? w.qacc[c] : 0.0f;
+          const float prevv = __uint_as_float(wave_shr1(__float_as_uint(pv), __float_as_uint(bv)));
+          if (closes) w.qacc[c] = q0 + (pv - prevv) * __int_as_float(se.x);
           const int n = __popcll(__ballot(closes));
-          if (n > 0)
-            last_end = __hiloint2double(readlane(eb.y, n - 1), readlane(eb.x, n - 1));
+          if (n > 0) bv = readlanef(pv, n - 1);
           cur += n;
           if (n < WAVE) break;
           se = *reinterpret_cast<const int2*>(&w.prm[min(cur + lane, r)].z);
         }
-        run += (double)readlanef(incl, WAVE - 1);
+        open_sum = readlanef(incl, WAVE - 1) - bv;
         wave_lds_sync();
       }
     };
@@ -374,7 +373,7 @@ __device__ __forceinline__ void step_body(const DevBasis& B, const float* __rest
     for (int c = threadIdx.x; c < r; c += blockDim.x) {
       double s = 0.0;
       for (int wv = 0; wv < WAVES_PER_WG; wv++)
-        s += carve_wave(lds + step_shared_lds_bytes(nk) + (size_t)wv * step_wave_lds_bytes(r), r).qacc[c];
+        s += (double)carve_wave(lds + step_shared_lds_bytes(nk) + (size_t)wv * step_wave_lds_bytes(r), r).qacc[c];
       partials[(size_t)wg * r + c] = s;
     }
   }
