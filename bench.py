@@ -259,6 +259,24 @@ def mfma_mops_from_profiles(workload: str):
     return float(m.group(1).replace(",", "")) if m else None
 
 
+def host_cpu():
+    """The host CPU's model name, its logical CPUs and those this process may use."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count()
+    return {"model": model, "logical_cpus": os.cpu_count(), "usable_cpus": usable}
+
+
 def cpu_baseline(bufs, v, q, r, sample_cols):
     """The FP64 oracle (single thread) on columns [0, sample_cols); a partial
     sample is extrapolated linearly in the column count to the full step."""
@@ -462,16 +480,23 @@ def main():
         threads = min(args.threads, os.cpu_count() or 1)
         from oracle import oracle as O
         ob = O.Basis(buffers=bufs)
-        t0 = time.perf_counter()
-        q_mt = ob.project_mt(v, threads)
-        f_mt = ob.unproject_mt(q, np.zeros_like(v), threads)
-        t_mt = time.perf_counter() - t0
+        times = []
+        for _ in range(4):  # one warm-up, then the median of three
+            t0 = time.perf_counter()
+            q_mt = ob.project_mt(v, threads)
+            f_mt = ob.unproject_mt(q, np.zeros_like(v), threads)
+            times.append(time.perf_counter() - t0)
+        t_mt = float(np.median(times[1:]))
         ob.close()
         same = ns == r and np.array_equal(q_mt, q_s) and np.array_equal(f_mt, f_s)
         result["cpu_baseline_all_cores"] = {
             "value": round(1.0 / t_mt, 5), "unit": "steps/s", "cores": threads, "kind": "port",
-            "sample": f"the same FP64 step on {threads} threads ({t_mt:.2f} s), results identical: {bool(same)}",
+            "sample": f"the same FP64 step on {threads} threads, median of 3 after one warm-up ({t_mt:.2f} s), "
+                      f"results identical: {bool(same)}",
         }
+        host = host_cpu()
+        result["cpu_baseline"]["host"] = host
+        result["cpu_baseline_all_cores"]["host"] = host
         log(f"[bench] CPU oracle step: 1 thread {t_full:.2f} s, {threads} threads {t_mt:.2f} s")
         # parity of the timed GPU step on the sampled columns
         zq = DeviceArray.from_numpy(np.concatenate([q[:ns], np.zeros(r - ns)]))
