@@ -221,6 +221,10 @@ __device__ __forceinline__ void step_body(const DevBasis& B, const float* __rest
     const int len = (int)it.d0.z;
     const uint64_t tbase = (uint64_t)it.d0.x | ((uint64_t)it.d0.y << 32);
     const uint32_t l0 = it.d1.z, l1 = it.d1.w;
+    // the item's first token chunks and long run, in flight during the
+    // segment set-up and the forward DCT below
+    MarkedStream<NT> parser(B.tokens + tbase, B.len_bits + (size_t)it.d2.w * (CH / 8), len);
+    int next_lr = l0 < l1 ? (int)B.lr[l0] : NO_END;
     float inv_scale = 0.0f, S = 0.0f;
     if (UNPROJ) {
       const float bound = __uint_as_float(it.d2.x) * qmax * 1.0001f;
@@ -243,8 +247,6 @@ __device__ __forceinline__ void step_body(const DevBasis& B, const float* __rest
     if (PROJ) forward_dct_rows(it.rows, zig, w.pfx, w.vz);
     wave_lds_sync();
 
-    MarkedStream<NT> parser(B.tokens + tbase, B.len_bits + (size_t)it.d2.w * (CH / 8), len);
-    int next_lr = l0 < l1 ? (int)B.lr[l0] : NO_END;
     uint32_t li = l0;
     float open_sum = 0.0f;  // project sum of segment cur's tokens in earlier chunks (wave-uniform)
     int cur = 0;            // first segment not yet closed
