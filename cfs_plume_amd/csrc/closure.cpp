@@ -32,6 +32,7 @@ struct ClosureNet {
   int r, rp, H;
 };
 hipError_t launch_advect(const float* vel, float* out, int X, int Y, int Z, float dt0, hipStream_t s);
+hipError_t launch_zero_words(void* p, long nwords, hipStream_t s);
 hipError_t launch_axpb(double* q, int r, double alpha, const double* forcing, hipStream_t s);
 hipError_t launch_mlp_fwd(const float* X, int ldx, int n, const uint16_t* Wp, const float* bias, int K, int M,
                           float* Z, float* Y, int act, hipStream_t s);
@@ -308,7 +309,7 @@ int cfs_closure_gradients(cfs_closure* c, const float* x, const float* t, int n,
   if (!c || !x || !t) return fail(CFS_ERR_ARG, "null argument");
   if (n < 1 || n > c->max_batch) return fail(CFS_ERR_ARG, "batch size outside [1, max_batch]");
   const hipStream_t s = (hipStream_t)stream;
-  HIP_TRY(hipMemsetAsync(c->loss.p, 0, 8, s), "hipMemsetAsync");
+  HIP_TRY(launch_zero_words(c->loss.p, 2, s), "zero launch");
   int st = c->forward(x, n, s);
   if (st) return st;
   if ((st = c->backward(x, t, n, s))) return st;
@@ -327,7 +328,7 @@ int cfs_closure_train_step(cfs_closure* c, const float* x, const float* t, const
   float* tb = c->tb.as<float>();
   HIP_TRY(launch_gather_rows(x, rows, n, c->rp, xb, s), "gather launch");
   HIP_TRY(launch_gather_rows(t, rows, n, c->rp, tb, s), "gather launch");
-  HIP_TRY(hipMemsetAsync(c->loss.p, 0, 8, s), "hipMemsetAsync");
+  HIP_TRY(launch_zero_words(c->loss.p, 2, s), "zero launch");
   int st = c->forward(xb, n, s);
   if (st) return st;
   if ((st = c->backward(xb, tb, n, s))) return st;
@@ -367,7 +368,7 @@ int cfs_reduced_advect(cfs_basis* U, const double* q_in, double* q_out, float* f
   if (st) return st;
   const int X = info.dims[0] + 2, Y = info.dims[1] + 2, Z = info.dims[2] + 2;
   const hipStream_t s = (hipStream_t)stream;
-  HIP_TRY(hipMemsetAsync(field_a, 0, (size_t)3 * X * Y * Z * 4, s), "hipMemsetAsync");
+  HIP_TRY(launch_zero_words(field_a, (long)3 * X * Y * Z, s), "zero launch");
   if ((st = cfs_unproject(U, q_in, field_a, stream))) return st;
   if ((st = cfs_advect_field(field_a, field_b, X, Y, Z, dt0, stream))) return st;
   if ((st = cfs_project(U, field_b, q_out, stream))) return st;

@@ -17,6 +17,7 @@
 // per workgroup.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 
 #include "mfma_idct.h"
@@ -55,6 +56,13 @@ __global__ void __launch_bounds__(256) advect_kernel(const float* __restrict__ v
 }
 
 // q_out = alpha * q_out + forcing (forcing may be null)
+// Clears n 32-bit words.  Used instead of hipMemsetAsync on the per-step
+// paths, so that a caller who captures them in a HIP graph records kernel
+// nodes only.
+__global__ void __launch_bounds__(256) zero_words_kernel(uint32_t* __restrict__ p, long n) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) p[i] = 0u;
+}
+
 __global__ void axpb_kernel(double* __restrict__ q, int r, double alpha, const double* __restrict__ forcing) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j < r) q[j] = alpha * q[j] + (forcing ? forcing[j] : 0.0);
@@ -325,6 +333,13 @@ __global__ void __launch_bounds__(CL_H * KS) closure_step_kernel(ClosureNet net,
 hipError_t launch_advect(const float* vel, float* out, int X, int Y, int Z, float dt0, hipStream_t s) {
   const int n = X * Y * Z;
   hipLaunchKernelGGL(advect_kernel, dim3((n + 255) / 256), dim3(256), 0, s, vel, out, X, Y, Z, dt0);
+  return hipGetLastError();
+}
+
+hipError_t launch_zero_words(void* p, long nwords, hipStream_t s) {
+  if (nwords <= 0) return hipSuccess;
+  const long blocks = std::min<long>((nwords + 255) / 256, 4096);
+  hipLaunchKernelGGL(zero_words_kernel, dim3((unsigned)blocks), dim3(256), 0, s, reinterpret_cast<uint32_t*>(p), nwords);
   return hipGetLastError();
 }
 

@@ -214,3 +214,51 @@ def test_closure_tracks_held_out_runs(plume):
     # with a check every step the fallback keeps each step within 1 % of the exact step
     assert rel(checked[-1], exact[0, -1]) <= 0.05
     assert 0 <= falls <= steps
+
+
+def test_per_step_paths_replay_in_a_hip_graph(small_basis):
+    """The exact reduced advection and the closure's gradient pass (both
+    clear device memory with a fill kernel, not a memset node) captured in a
+    HIP graph: every replay, with eager work and synchronisations between the
+    replays, equals the eager result."""
+    from cfs_plume_amd import _lib
+    from cfs_plume_amd.hip import Graph, Stream
+    ob, gb = small_basis
+    X, Y, Z = gb.dims
+    n = 3 * (X + 2) * (Y + 2) * (Z + 2)
+    rng = np.random.default_rng(17)
+    dq = DeviceArray.from_numpy(rng.normal(0, 40.0, gb.r))
+    df = DeviceArray.from_numpy(rng.normal(0, 1.0, gb.r))
+    fa, fb = DeviceArray((n,), np.float32), DeviceArray((n,), np.float32)
+    q_e, q_g = DeviceArray((gb.r,), np.float64), DeviceArray((gb.r,), np.float64)
+    r, nb = 52, 40
+    cl, _ = _net(r, 23)
+    x = DeviceArray.from_numpy(cl.pad(rng.normal(size=(nb, r))))
+    t = DeviceArray.from_numpy(cl.pad(rng.normal(size=(nb, r))))
+    g_e, g_g = DeviceArray((cl.n_params,), np.float32), DeviceArray((cl.n_params,), np.float32)
+    l_e, l_g = DeviceArray((1,), np.float64), DeviceArray((1,), np.float64)
+    s = Stream()
+    lib = _lib.lib()
+
+    def work(q_out, grads, loss):
+        reduced_advect(gb, dq, q_out, fa, fb, 0.8, 0.97, df, s)
+        _lib.check(lib.cfs_closure_gradients(cl.handle, x.ptr, t.ptr, nb, grads.ptr, loss.ptr, s.handle),
+                   "cfs_closure_gradients")
+
+    print("[case] graph replay of reduced advect r=6 and closure gradients n=40", flush=True)
+    work(q_e, g_e, l_e)
+    s.synchronize()
+    eager = (q_e.numpy(), g_e.numpy(), l_e.numpy())
+    graph = Graph(s, lambda: work(q_g, g_g, l_g))
+    for rep in range(3):
+        for a in (q_g, g_g, l_g):
+            a.fill_zero()
+        graph.launch()
+        s.synchronize()
+        assert np.array_equal(q_g.numpy(), eager[0]), f"replay {rep}: q"
+        assert rel(g_g.numpy(), eager[1]) <= 1e-6, f"replay {rep}: gradients"
+        assert rel(l_g.numpy(), eager[2]) <= 1e-12, f"replay {rep}: loss"
+        work(q_e, g_e, l_e)  # eager work between the replays
+        s.synchronize()
+        assert np.array_equal(q_e.numpy(), eager[0])
+    cl.close()
