@@ -277,3 +277,32 @@ def test_submatrix(case, request):
         assert rel(got, want) <= TOL, (cell, rel(got, want))
 
 
+
+
+def test_fused_step_replays_in_a_hip_graph(smooth):
+    """cfs_step issues kernel launches only: captured in a HIP graph, every
+    replay (with eager steps and synchronisations between the replays) equals
+    the eager step bit for bit."""
+    from cfs_plume_amd.hip import Graph, Stream
+    ob, gb = smooth
+    v, q = _fields(ob, 45)
+    dv = DeviceArray.from_numpy(v.astype(np.float32))
+    dq = DeviceArray.from_numpy(q)
+    q_e, q_g = DeviceArray.zeros((ob.r,), np.float64), DeviceArray.zeros((ob.r,), np.float64)
+    f_e, f_g = DeviceArray.zeros((len(v),), np.float32), DeviceArray.zeros((len(v),), np.float32)
+    s = Stream()
+    step(gb, gb, dv, q_e, dq, f_e, s)
+    s.synchronize()
+    eager = (q_e.numpy(), f_e.numpy())
+    graph = Graph(s, lambda: step(gb, gb, dv, q_g, dq, f_g, s))
+    for rep in range(3):
+        q_g.fill_zero()
+        f_g.fill_zero()
+        graph.launch()
+        s.synchronize()
+        assert np.array_equal(q_g.numpy(), eager[0]), f"replay {rep}: q"
+        assert np.array_equal(f_g.numpy(), eager[1]), f"replay {rep}: field"
+        step(gb, gb, dv, q_e, dq, f_e, s)
+        s.synchronize()
+        assert np.array_equal(q_e.numpy(), eager[0])
+    assert gb.check_stream_errors() == 0
