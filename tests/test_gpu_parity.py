@@ -3,7 +3,7 @@
 Integer stages (RLE tokens -> zigzag -> ints) must match bit for bit.  FP32
 results are compared with relative L2 error against the oracle's FP64:
   decoded coefficients / columns / submatrix  <= 1e-5
-  projected q, unprojected field              <= 1e-5   (north-star bound)
+  projected q, unprojected field              <= 1e-5   (the step's accuracy bound)
 """
 import numpy as np
 import pytest
