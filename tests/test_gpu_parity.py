@@ -147,7 +147,18 @@ def wide(tmp_path_factory, gpu):
     return _pair(H.basis_from_ints(dims, 256, wide_zz, seed=14), tmp_path_factory, "wide")
 
 
-CASES = ["smooth", "rough", "adversarial", "hazard", "runs", "wide"]
+@pytest.fixture(scope="module")
+def ragged(tmp_path_factory, gpu):
+    # peeled dims that are not multiples of 8 on any axis (partial blocks on
+    # every face) and 130 columns (three column slots per lane, the last
+    # partly filled)
+    dims = (23, 13, 9)
+    U = synth.synthetic_basis(dims, 130, seed=8)
+    print(f"[case] ragged basis dims={dims} r=130", flush=True)
+    return _pair(encode_matrix(U, dims).buffers(), tmp_path_factory, "ragged")
+
+
+CASES = ["smooth", "rough", "adversarial", "hazard", "runs", "wide", "ragged"]
 
 
 def test_hazards_are_listed(hazard):
