@@ -141,7 +141,8 @@ def run_batch(args):
             "sequential_ms_per_step": round(seq_ms, 5),
         },
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None},
+                     "frac": round(achieved / PEAK_HBM_GBS, 4),
+                     "traffic": traffic_from_profiles(BATCH, "batch_step_kernel")},
     }
     print(json.dumps(result), flush=True)
 
@@ -240,9 +241,9 @@ def run_closure(args):
     print(json.dumps(result), flush=True)
 
 
-def traffic_from_profiles(workload: str):
-    """HBM bytes per step kernel launch from the committed PMC summary, if any."""
-    path = os.path.join(ROOT, "profiles", f"step_kernel_{workload}_pmc.md")
+def traffic_from_profiles(workload: str, kernel: str = "step_kernel"):
+    """HBM bytes per launch of `kernel` from the committed PMC summary, if any."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_pmc.md")
     if not os.path.exists(path):
         return None
     m = re.search(r"hbm_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read())
