@@ -1,10 +1,11 @@
 // closure.cpp -- C ABI of the reduced-advection step and its learned closure.
 //
 // cfs_reduced_advect: q' = alpha U^T A(U q) + f, the exact reduced
-// semi-Lagrangian advection on the full grid (the closure's fallback and the
-// reference its accuracy is measured against; the reference approximates the same projection with cubature key
-// cells, reducedAdvectCompressionFriendly, SUBSPACE_FLUID_3D_COMPRESSED_EIGEN.cpp:
-// 1781-1854, and advects with VECTOR3_FIELD_3D::advect, VECTOR3_FIELD_3D.cpp:
+// semi-Lagrangian advection on the full grid.  It is the closure's fallback
+// and the reference its accuracy is measured against.  The reference
+// approximates the same projection with cubature key cells
+// (reducedAdvectCompressionFriendly, SUBSPACE_FLUID_3D_COMPRESSED_EIGEN.cpp:
+// 1781-1854) and advects with VECTOR3_FIELD_3D::advect (VECTOR3_FIELD_3D.cpp:
 // 2001-2077).
 //
 // cfs_closure_*: the MLP r -> 256 -> 256 -> 256 -> r (GELU) predicting the
