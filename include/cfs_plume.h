@@ -48,7 +48,9 @@ typedef struct {
   int64_t device_bytes;  /* device memory held by the handle */
 } cfs_basis_info;
 
-/* Message for the last failing call on this thread ("" if none). */
+/* Message for the last failing call on this thread ("" if none); with
+ * cfs_abi_version, the library's own bookkeeping (no counterpart in the
+ * reference, which prints and returns NULL or asserts). */
 const char* cfs_last_error(void);
 int cfs_abi_version(void);
 
@@ -109,7 +111,9 @@ int cfs_decode_block_coeffs(cfs_basis* basis, int comp, int col, int block0, int
 int cfs_get_submatrix(cfs_basis* basis, int start_row, double* out, void* stream);
 
 /* Synchronises, returns the CFS_STREAM_* bits raised since the last call and
- * clears them. */
+ * clears them.  No counterpart in the reference, whose decoder asserts on a
+ * malformed stream (COMPRESSION.cpp:1206-1207): a kernel records the fault
+ * here instead. */
 int cfs_check_stream_errors(cfs_basis* basis, uint32_t* bits);
 
 /* ---- batched step: several bases, one persistent launch -------------------

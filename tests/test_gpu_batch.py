@@ -21,12 +21,14 @@ def rel(a, b):
 
 @pytest.fixture(scope="module")
 def bases(gpu):
-    print("[case] batch of 4 bases: (30,22,14) r8, (14,14,6) r3, (17,9,8) r5 adversarial, (16,16,8) r6 runs", flush=True)
+    print("[case] batch of 5 bases: (30,22,14) r8, (14,14,6) r3, (17,9,8) r5 adversarial, (16,16,8) r6 runs, "
+          "(5,3,2) r1", flush=True)
     bufs = [
         encode_matrix(synth.synthetic_basis((30, 22, 14), 8, seed=3), (30, 22, 14)).buffers(),
         encode_matrix(synth.synthetic_basis((14, 14, 6), 3, seed=5), (14, 14, 6)).buffers(),
         H.basis_from_ints((17, 9, 8), 5, adversarial_zz, seed=11),
         H.basis_from_ints((16, 16, 8), 6, runs_zz, seed=13),
+        encode_matrix(synth.synthetic_basis((5, 3, 2), 1, seed=9), (5, 3, 2)).buffers(),
     ]
     return [(O.Basis(buffers=b), CompressedBasis.from_buffers(b)) for b in bufs]
 

@@ -158,7 +158,17 @@ def ragged(tmp_path_factory, gpu):
     return _pair(encode_matrix(U, dims).buffers(), tmp_path_factory, "ragged")
 
 
-CASES = ["smooth", "rough", "adversarial", "hazard", "runs", "wide", "ragged"]
+@pytest.fixture(scope="module")
+def tiny(tmp_path_factory, gpu):
+    # one partial block (5 x 3 x 2 cells of an 8^3 block) and one column:
+    # the smallest basis the format holds
+    dims = (5, 3, 2)
+    U = synth.synthetic_basis(dims, 1, seed=9)
+    print(f"[case] tiny basis dims={dims} r=1", flush=True)
+    return _pair(encode_matrix(U, dims).buffers(), tmp_path_factory, "tiny")
+
+
+CASES = ["smooth", "rough", "adversarial", "hazard", "runs", "wide", "ragged", "tiny"]
 
 
 def test_hazards_are_listed(hazard):
