@@ -37,6 +37,7 @@ WORKLOADS = {
 DEFAULT = "128x192x128_r200"
 BATCH = "batch16"  # 16 independent bases of seeded sizes in one persistent launch
 CLOSURE = "closure"  # learned advection closure vs the exact reduced-advection step, 48x64x48 r=52
+ACCURACY = "accuracy"  # compressed vs uncompressed reduced plume, per-step q error, 48x64x48 r=52
 
 
 def batch_specs(n=16, seed=2024):
@@ -241,6 +242,48 @@ def run_closure(args):
     print(json.dumps(result), flush=True)
 
 
+def run_accuracy(args):
+    """Compression error of the reduced plume (the measurement of the
+    reference's reducedCompressed3DCompareVelocities): the 48x64x48, r = 52
+    basis coded at percent 0.9995, seeded runs stepped 100 times with the
+    compressed basis on the device and with the dense basis it was encoded
+    from; per-step relative L2 error of q."""
+    import cfs_plume_amd as C
+    from cfs_plume_amd import accuracy as A
+    from cfs_plume_amd import synth
+
+    grid, r, percent, steps = (48, 64, 48), 52, 0.9995, 100
+    dims = tuple(g - 2 for g in grid)
+    U_dense = synth.synthetic_basis(dims, r, seed=7)
+    bufs = load_or_encode(C, synth, grid, r, percent, 7, "48x64x48_r52", args.cache, args.threads)
+    U = C.CompressedBasis.from_buffers(bufs)
+    plume = C.ReducedPlume(U, dt0=1.0, alpha=0.99, force=25.0)
+    runs = C.ReducedPlume.seeded_runs(args.accuracy_runs, seed=2026)
+    t0 = time.perf_counter()
+    errs = A.compare_runs(plume, U_dense, runs, steps, out_prefix=args.accuracy_out or None)
+    mean = errs.mean(axis=0)
+    log(f"[bench] compressed vs uncompressed q over {len(runs)} runs ({time.perf_counter() - t0:.1f} s): "
+        f"step 1 {mean[0]:.3%}, step 50 {mean[49]:.3%}, step 100 {mean[99]:.3%}, max {errs.max():.3%}")
+    result = {
+        "metric": "compressed vs uncompressed reduced q, relative L2 at step 100 (r=52 @48x64x48, percent 0.9995)",
+        "value": round(float(mean[steps - 1]), 6), "unit": "relative L2", "n_gpus": 1, "steps": steps,
+        "warmup": 0, "ms_per_step": None, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32 fields, fp64 q", "data": "synthetic",
+        "config": {
+            "workload": f"reduced plume stand-in q' = 0.99 U^T A(U q) + U^T f, {len(runs)} seeded runs of {steps} "
+                        f"steps, compressed (device) against dense FP64 basis",
+            "grid": list(grid), "r": r, "percent": percent, "parallelism": "single GPU",
+        },
+        "accuracy": {
+            "mean_rel_l2": {"1": round(float(mean[0]), 6), "10": round(float(mean[9]), 6),
+                            "50": round(float(mean[49]), 6), "100": round(float(mean[99]), 6)},
+            "max_rel_l2": round(float(errs.max()), 6),
+            "reference_readme_rel_l2": {"1": 0.0025, "50": 0.0091, "100": 0.0106},
+        },
+    }
+    print(json.dumps(result), flush=True)
+
+
 def traffic_from_profiles(workload: str, kernel: str = "step_kernel"):
     """HBM bytes per launch of `kernel` from the committed PMC summary, if any."""
     path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_pmc.md")
@@ -301,7 +344,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default=DEFAULT, choices=sorted(WORKLOADS) + [BATCH, CLOSURE])
+    ap.add_argument("--workload", default=DEFAULT, choices=sorted(WORKLOADS) + [BATCH, CLOSURE, ACCURACY])
+    ap.add_argument("--accuracy-runs", type=int, default=4, help="seeded runs of the accuracy workload")
+    ap.add_argument("--accuracy-out", default="",
+                    help="write each accuracy run's trajectories as <prefix>.<i>[.compressed].qDot.matrix")
     ap.add_argument("--closure-runs", type=int, default=200, help="training runs of the closure workload")
     ap.add_argument("--closure-epochs", type=int, default=200)
     ap.add_argument("--closure-noise", type=float, default=0.1,
@@ -327,6 +373,9 @@ def main():
         return
     if args.workload == CLOSURE:
         run_closure(args)
+        return
+    if args.workload == ACCURACY:
+        run_accuracy(args)
         return
 
     import cfs_plume_amd as C
