@@ -36,6 +36,16 @@ def advect_field(vel: DeviceArray, out: DeviceArray, full_dims, dt0: float, stre
     return out
 
 
+def advect_scalar(vel: DeviceArray, old: DeviceArray, out: DeviceArray, full_dims, dt0: float,
+                  stream: Optional[Stream] = None):
+    """out = VECTOR3_FIELD_3D::advect of the full-grid scalar field `old` along
+    vel (the density and heat advection of advectHeatAndDensityStam)."""
+    X, Y, Z = full_dims
+    _lib.check(_lib.lib().cfs_advect_scalar(vel.ptr, old.ptr, out.ptr, X, Y, Z, float(dt0), _sptr(stream)),
+               "cfs_advect_scalar")
+    return out
+
+
 def _ptr(a: Optional[DeviceArray], offset_elems: int = 0):
     if a is None:
         return None

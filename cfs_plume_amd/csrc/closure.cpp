@@ -33,6 +33,8 @@ struct ClosureNet {
   int r, rp, H;
 };
 hipError_t launch_advect(const float* vel, float* out, int X, int Y, int Z, float dt0, hipStream_t s);
+hipError_t launch_advect_scalar(const float* vel, const float* old, float* out, int X, int Y, int Z, float dt0,
+                                hipStream_t s);
 hipError_t launch_zero_words(void* p, long nwords, hipStream_t s);
 hipError_t launch_axpb(double* q, int r, double alpha, const double* forcing, hipStream_t s);
 hipError_t launch_mlp_fwd(const float* X, int ldx, int n, const uint16_t* Wp, const float* bias, int K, int M,
@@ -358,6 +360,15 @@ int cfs_advect_field(const float* vel, float* out, int X, int Y, int Z, float dt
   if (X < 4 || Y < 4 || Z < 4) return fail(CFS_ERR_ARG, "full-grid dims must be at least 4");
   if ((double)X * Y * Z * 3 >= 2147483647.0) return fail(CFS_ERR_ARG, "grid too large");
   HIP_TRY(launch_advect(vel, out, X, Y, Z, dt0, (hipStream_t)stream), "advect launch");
+  return CFS_OK;
+}
+
+int cfs_advect_scalar(const float* vel, const float* old, float* out, int X, int Y, int Z, float dt0, void* stream) {
+  if (!vel || !old || !out) return fail(CFS_ERR_ARG, "null argument");
+  if (old == out) return fail(CFS_ERR_ARG, "the advected field must not be written in place");
+  if (X < 4 || Y < 4 || Z < 4) return fail(CFS_ERR_ARG, "full-grid dims must be at least 4");
+  if ((double)X * Y * Z * 3 >= 2147483647.0) return fail(CFS_ERR_ARG, "grid too large");
+  HIP_TRY(launch_advect_scalar(vel, old, out, X, Y, Z, dt0, (hipStream_t)stream), "advect launch");
   return CFS_OK;
 }
 

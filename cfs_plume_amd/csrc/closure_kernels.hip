@@ -55,6 +55,33 @@ __global__ void __launch_bounds__(256) advect_kernel(const float* __restrict__ v
   }
 }
 
+// semi-Lagrangian advection of a full-grid scalar field (density, heat) by
+// an interleaved velocity field (VECTOR3_FIELD_3D::advect for a FIELD_3D,
+// VECTOR3_FIELD_3D.cpp:2082-2160): the same backtrace and clamp as above,
+// one trilinear sample per cell
+__global__ void __launch_bounds__(256) advect_scalar_kernel(const float* __restrict__ vel,
+                                                            const float* __restrict__ old, float* __restrict__ out,
+                                                            int X, int Y, int Z, float dt0) {
+  const int n = X * Y * Z;
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n) return;
+  const int x = idx % X, y = (idx / X) % Y, z = idx / (X * Y);
+  float xt = (float)x - dt0 * vel[3 * idx];
+  float yt = (float)y - dt0 * vel[3 * idx + 1];
+  float zt = (float)z - dt0 * vel[3 * idx + 2];
+  xt = fminf(fmaxf(xt, 1.5f), (float)X - 2.5f);
+  yt = fminf(fmaxf(yt, 1.5f), (float)Y - 2.5f);
+  zt = fminf(fmaxf(zt, 1.5f), (float)Z - 2.5f);
+  const int x0 = (int)xt, y0 = (int)yt, z0 = (int)zt;
+  const float s1 = xt - (float)x0, s0 = 1.0f - s1;
+  const float t1 = yt - (float)y0, t0 = 1.0f - t1;
+  const float u1 = zt - (float)z0, u0 = 1.0f - u1;
+  const int slab = X * Y;
+  const float* f = old + x0 + y0 * X + z0 * slab;
+  out[idx] = u0 * (s0 * (t0 * f[0] + t1 * f[X]) + s1 * (t0 * f[1] + t1 * f[X + 1])) +
+             u1 * (s0 * (t0 * f[slab] + t1 * f[slab + X]) + s1 * (t0 * f[slab + 1] + t1 * f[slab + X + 1]));
+}
+
 // q_out = alpha * q_out + forcing (forcing may be null)
 // Clears n 32-bit words.  Used instead of hipMemsetAsync on the per-step
 // paths, so that a caller who captures them in a HIP graph records kernel
@@ -333,6 +360,13 @@ __global__ void __launch_bounds__(CL_H * KS) closure_step_kernel(ClosureNet net,
 hipError_t launch_advect(const float* vel, float* out, int X, int Y, int Z, float dt0, hipStream_t s) {
   const int n = X * Y * Z;
   hipLaunchKernelGGL(advect_kernel, dim3((n + 255) / 256), dim3(256), 0, s, vel, out, X, Y, Z, dt0);
+  return hipGetLastError();
+}
+
+hipError_t launch_advect_scalar(const float* vel, const float* old, float* out, int X, int Y, int Z, float dt0,
+                                hipStream_t s) {
+  const int n = X * Y * Z;
+  hipLaunchKernelGGL(advect_scalar_kernel, dim3((n + 255) / 256), dim3(256), 0, s, vel, old, out, X, Y, Z, dt0);
   return hipGetLastError();
 }
 

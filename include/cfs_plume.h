@@ -158,6 +158,11 @@ int cfs_reduced_advect(cfs_basis* U, const double* q_in, double* q_out, float* f
                        double alpha, const double* forcing, void* stream);
 /* VECTOR3_FIELD_3D::advect of a full-grid field by itself (dims >= 4). */
 int cfs_advect_field(const float* vel, float* out, int X, int Y, int Z, float dt0, void* stream);
+/* VECTOR3_FIELD_3D::advect of a full-grid scalar field (VECTOR3_FIELD_3D.cpp:2082-2160,
+ * the density and heat advection of advectHeatAndDensityStam,
+ * SUBSPACE_FLUID_3D_COMPRESSED_EIGEN.cpp:796-824): out = old sampled at the
+ * backtrace of each cell along vel (FP32, X*Y*Z, x fastest); out != old. */
+int cfs_advect_scalar(const float* vel, const float* old, float* out, int X, int Y, int Z, float dt0, void* stream);
 
 /* Learned closure (no counterpart in the reference): an MLP r -> 256 -> 256 ->
  * 256 -> r with GELU, BF16 weights (FP32 master copies) and FP32

@@ -8,6 +8,8 @@ advect():          VECTOR3_FIELD_3D::advect (reference VECTOR3_FIELD_3D.cpp:2001
                    interleaved field: backtrace by dt0 * v, clamp to
                    [1.5, res - 2.5], trilinear interpolation in the reference's
                    evaluation order.
+advect_scalar():   the same for a scalar field along a velocity field
+                   (VECTOR3_FIELD_3D.cpp:2082-2160).
 reduced_advect():  the quantity the cubature step reducedAdvectCompressionFriendly
                    (SUBSPACE_FLUID_3D_COMPRESSED_EIGEN.cpp:1781-1854) approximates:
                    U^T A(U q) with the FP64 decoder (oracle.Basis).
@@ -17,7 +19,9 @@ mlp_*():           the closure network r -> 256 -> 256 -> 256 -> r with GELU
 
 Pins (tests/test_closure_oracle.py): the backward pass against central finite
 differences, GELU' against finite differences, and advect() against closed
-forms (a linear field under a uniform velocity, a constant field, dt0 = 0).
+forms (a linear field under a uniform velocity, a constant field, dt0 = 0);
+advect_scalar() against advect() component by component and against a
+linear density under a uniform velocity.
 """
 from __future__ import annotations
 
@@ -49,6 +53,28 @@ def advect(vel: np.ndarray, full_dims, dt0: float) -> np.ndarray:
 
         out[..., c] = (u0 * (s0 * (t0 * at(0, 0, 0) + t1 * at(0, 1, 0)) + s1 * (t0 * at(1, 0, 0) + t1 * at(1, 1, 0))) +
                        u1 * (s0 * (t0 * at(0, 0, 1) + t1 * at(0, 1, 1)) + s1 * (t0 * at(1, 0, 1) + t1 * at(1, 1, 1))))
+    return out.reshape(-1)
+
+
+def advect_scalar(vel: np.ndarray, old: np.ndarray, full_dims, dt0: float) -> np.ndarray:
+    """VECTOR3_FIELD_3D::advect for a FIELD_3D (reference VECTOR3_FIELD_3D.cpp:2082-2160):
+    the scalar field `old` sampled trilinearly at each cell's clamped backtrace along vel."""
+    X, Y, Z = full_dims
+    v = np.asarray(vel, np.float64).reshape(Z, Y, X, 3)
+    f = np.asarray(old, np.float64).reshape(Z, Y, X)
+    z, y, x = np.meshgrid(np.arange(Z), np.arange(Y), np.arange(X), indexing="ij")
+    xt = np.clip(x - dt0 * v[..., 0], 1.5, X - 2.5)
+    yt = np.clip(y - dt0 * v[..., 1], 1.5, Y - 2.5)
+    zt = np.clip(z - dt0 * v[..., 2], 1.5, Z - 2.5)
+    x0, y0, z0 = xt.astype(np.int64), yt.astype(np.int64), zt.astype(np.int64)
+    s1, t1, u1 = xt - x0, yt - y0, zt - z0
+    s0, t0, u0 = 1.0 - s1, 1.0 - t1, 1.0 - u1
+
+    def at(dx, dy, dz):
+        return f[z0 + dz, y0 + dy, x0 + dx]
+
+    out = (u0 * (s0 * (t0 * at(0, 0, 0) + t1 * at(0, 1, 0)) + s1 * (t0 * at(1, 0, 0) + t1 * at(1, 1, 0))) +
+           u1 * (s0 * (t0 * at(0, 0, 1) + t1 * at(0, 1, 1)) + s1 * (t0 * at(1, 0, 1) + t1 * at(1, 1, 1))))
     return out.reshape(-1)
 
 

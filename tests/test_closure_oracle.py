@@ -126,3 +126,32 @@ def test_advect_constant_and_zero_step():
     out = C.advect(f, full, 0.0).reshape(Z, Y, X, 3)
     f = f.reshape(Z, Y, X, 3)
     np.testing.assert_array_equal(out[2:Z - 2, 2:Y - 2, 2:X - 2], f[2:Z - 2, 2:Y - 2, 2:X - 2])
+
+
+def test_advect_scalar_agrees_with_the_vector_advection():
+    """advect_scalar (VECTOR3_FIELD_3D.cpp:2082-2160) of one component of a
+    field along that same field equals that component of the self-advection
+    (VECTOR3_FIELD_3D.cpp:2001-2077), whose closed forms are pinned above."""
+    full = (9, 8, 7)
+    X, Y, Z = full
+    v = np.random.default_rng(5).standard_normal(3 * X * Y * Z) * 0.8
+    vec = C.advect(v, full, 1.3).reshape(-1, 3)
+    for c in range(3):
+        s = C.advect_scalar(v, v.reshape(-1, 3)[:, c], full, 1.3)
+        np.testing.assert_allclose(s, vec[:, c], rtol=0, atol=1e-14)
+
+
+def test_advect_scalar_linear_density_closed_form():
+    """A density linear in x moved by a uniform x velocity u: inside the clamp
+    window the result is a (x - dt0 u) + b exactly."""
+    full = (12, 7, 6)
+    X, Y, Z = full
+    a, b, u, dt0 = 0.75, -2.0, 0.6, 1.5
+    z, y, x = np.meshgrid(np.arange(Z), np.arange(Y), np.arange(X), indexing="ij")
+    dens = (a * x + b).astype(np.float64)
+    vel = np.zeros((Z, Y, X, 3))
+    vel[..., 0] = u
+    out = C.advect_scalar(vel.reshape(-1), dens.reshape(-1), full, dt0).reshape(Z, Y, X)
+    xt = x - dt0 * u
+    inside = (xt >= 1.5) & (xt <= X - 2.5) & (y >= 2) & (y <= Y - 3) & (z >= 2) & (z <= Z - 3)
+    np.testing.assert_allclose(out[inside], (a * xt + b)[inside], rtol=0, atol=1e-12)

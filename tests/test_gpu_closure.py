@@ -262,3 +262,18 @@ def test_per_step_paths_replay_in_a_hip_graph(small_basis):
         s.synchronize()
         assert np.array_equal(q_e.numpy(), eager[0])
     cl.close()
+
+
+def test_advect_scalar_matches_oracle(gpu):
+    """cfs_advect_scalar against the FP64 oracle (VECTOR3_FIELD_3D.cpp:2082-2160)."""
+    from cfs_plume_amd import advect_scalar
+    full = (20, 16, 12)
+    X, Y, Z = full
+    rng = np.random.default_rng(8)
+    vel = (rng.standard_normal(3 * X * Y * Z) * 1.5).astype(np.float32)
+    dens = rng.random(X * Y * Z).astype(np.float32)
+    out = DeviceArray.zeros((X * Y * Z,), np.float32)
+    advect_scalar(DeviceArray.from_numpy(vel), DeviceArray.from_numpy(dens), out, full, 1.25)
+    want = C.advect_scalar(vel.astype(np.float64), dens.astype(np.float64), full, 1.25)
+    got = out.numpy().astype(np.float64)
+    assert np.max(np.abs(got - want)) <= 1e-5 * max(1.0, np.max(np.abs(want)))
