@@ -97,6 +97,11 @@ def test_closure_and_advection_entry_points_reject_bad_arguments():
     assert lib.cfs_advect_field(None, None, 8, 8, 8, 1.0, None) == -1
     buf = (ctypes.c_float * 3)()
     assert lib.cfs_advect_field(buf, buf, 3, 8, 8, 1.0, None) == -1   # dims < 4
+    buf2 = (ctypes.c_float * 3)()
+    assert lib.cfs_advect_scalar(None, buf, buf2, 8, 8, 8, 1.0, None) == -1
+    assert lib.cfs_advect_scalar(buf, buf2, buf2, 8, 8, 8, 1.0, None) == -1   # in place
+    assert "in place" in lib.cfs_last_error().decode()
+    assert lib.cfs_advect_scalar(buf, buf, buf2, 8, 3, 8, 1.0, None) == -1   # dims < 4
     assert lib.cfs_reduced_advect(None, None, None, None, None, 1.0, 1.0, None, None) == -1
     assert lib.cfs_closure_step(None, None, None, 1, 1.0, None, None) == -1
     assert "null" in lib.cfs_last_error().decode()
