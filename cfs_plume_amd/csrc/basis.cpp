@@ -81,14 +81,28 @@ int parse_file(const uint8_t* d, size_t n, int comp, FileView* f) {
   f->r = h[8]; f->nB = h[9];
   if (f->nBits < 1 || f->nBits > 32) return fail(CFS_ERR_FORMAT, who + "nBits out of range");
   if (f->X < 1 || f->Y < 1 || f->Z < 1) return fail(CFS_ERR_FORMAT, who + "bad dims");
-  if (f->Px != f->X + pad_of(f->X) || f->Py != f->Y + pad_of(f->Y) || f->Pz != f->Z + pad_of(f->Z))
+  // dims are positive int32, so X + 7 and the padded dims fit in int64
+  if ((int64_t)f->Px != (int64_t)f->X + pad_of(f->X) || (int64_t)f->Py != (int64_t)f->Y + pad_of(f->Y) ||
+      (int64_t)f->Pz != (int64_t)f->Z + pad_of(f->Z))
     return fail(CFS_ERR_FORMAT, who + "padded dims do not match dims");
-  if ((long long)f->nB != (long long)(f->Px / BS) * (f->Py / BS) * (f->Pz / BS))
-    return fail(CFS_ERR_FORMAT, who + "numBlocks does not match padded dims");
+  // (P/8)^3 < 2^84 can overflow 64 bits: multiply step by step against the int32 nB
+  {
+    const uint64_t bx = (uint64_t)(f->Px / BS), by = (uint64_t)(f->Py / BS), bz = (uint64_t)(f->Pz / BS);
+    const uint64_t lim = 0x7fffffffull;
+    const bool fits = bx * by <= lim && bx * by * bz <= lim;
+    if (f->nB < 1 || !fits || (uint64_t)f->nB != bx * by * bz)
+      return fail(CFS_ERR_FORMAT, who + "numBlocks does not match padded dims");
+  }
   if (f->r < 1) return fail(CFS_ERR_FORMAT, who + "numCols <= 0");
-  const size_t m = (size_t)f->r * f->nB;
+  if (f->r > step_max_cols())
+    return fail(CFS_ERR_FORMAT, who + "numCols " + std::to_string(f->r) + " exceeds the supported maximum " +
+                                    std::to_string(step_max_cols()));
+  // r <= 256 and nB < 2^31, so m * 24 < 2^44 cannot wrap; the size check
+  // divides instead of multiplying anyway
+  const size_t m = (size_t)f->r * (size_t)f->nB;
+  if ((n - 40) / 24 < m) return fail(CFS_ERR_FORMAT, who + "truncated metadata or token stream");
   const size_t meta = 40 + m * 24;
-  if (n < meta || (n - meta) % 4 != 0) return fail(CFS_ERR_FORMAT, who + "truncated metadata or token stream");
+  if ((n - meta) % 4 != 0) return fail(CFS_ERR_FORMAT, who + "truncated metadata or token stream");
   // the arrays are 8- or 4-byte aligned inside the image because the header is 40 bytes
   f->s = reinterpret_cast<const double*>(d + 40);
   f->gamma = f->s + m;
@@ -257,9 +271,6 @@ int build_basis(const uint8_t* const bufs[3], const size_t sizes[3], cfs_basis**
     int st = parse_file(bufs[k], sizes[k], k, &f[k]);
     if (st) return st;
   }
-  if (f[0].r > step_max_cols())
-    return fail(CFS_ERR_FORMAT, "numCols " + std::to_string(f[0].r) + " exceeds the supported maximum " +
-                                    std::to_string(step_max_cols()));
   for (int k = 1; k < 3; k++)
     if (f[k].nBits != f[0].nBits || f[k].X != f[0].X || f[k].Y != f[0].Y || f[k].Z != f[0].Z || f[k].r != f[0].r ||
         f[k].nB != f[0].nB)

@@ -411,6 +411,7 @@ hipError_t launch_decode(const DevBasis& B, int mode, const DecodeArgs& a, const
                          hipStream_t s) {
   if (mode == DEC_SPATIAL) {
     const long items = (long)((a.ncol + SP_G - 1) / SP_G) * B.nB;
+    if (a.ncol == 0) return hipSuccess;  // an empty column range, as the other decoders
     if (items <= 0 || a.k0 != 0 || a.nk3 != 3 || a.b0 != 0 || a.nblk != B.nB) return hipErrorInvalidValue;
     const size_t lds = spatial_lds_bytes(B.nk);
     int dev = 0, cus = 0, per = 0;

@@ -1,11 +1,13 @@
 """The reduced per-step matrices of stepPlume, read from the reference's files.
 
-SUBSPACE_FLUID_3D_COMPRESSED_EIGEN.cpp:170-205 reads, from the reduced path,
-projected.ptof.matrix (P, pre-projection to final), projected.vtod.matrix,
-projected.ptov.matrix, projected.A.matrix, damping.peeled.matrix (D) and
-inverseProduct.matrix; when the last is missing it is built as
-ptov * A^-1 * vtod and written.  Between the advection and the unprojection
-stepPlume (:453-501) applies
+initOutOfCore (SUBSPACE_FLUID_3D_COMPRESSED_EIGEN.cpp:84-139) requires, in the
+reduced path, projected.ptof.matrix (P, pre-projection to final),
+projected.vtod.matrix, projected.ptov.matrix, projected.A.matrix,
+damping.peeled.matrix (D) and inverseProduct.matrix, and exits ("run
+./buildProducts first") when any is missing.  When inverseProduct.matrix
+exists but cannot be read, it is rebuilt as ptov * A^-1 * vtod and written
+back.  Between the advection and the unprojection stepPlume (:453-501)
+applies
 
     reducedPeeledDiffusion  (:848-854)   q <- D q
     reducedStagedProject    (:942-946)   q <- P q + inverseProduct q
@@ -46,18 +48,19 @@ class ReducedOperators:
 
     @classmethod
     def load(cls, reduced_path: str) -> "ReducedOperators":
-        """Reads the matrices as the reference does; a missing inverseProduct
-        is built from ptov, A and vtod and written next to them.  Missing
-        inputs raise FileNotFoundError (the reference prints and exits)."""
+        """Reads the six matrices as initOutOfCore does (:84-139).  Any missing
+        file raises FileNotFoundError (the reference prints "run ./buildProducts
+        first" and exits); an inverseProduct.matrix that exists but does not
+        read as a matrix is rebuilt from ptov, A and vtod and written back."""
         paths = {k: os.path.join(reduced_path, f) for k, f in _FILES.items()}
-        missing = [p for p in paths.values() if not os.path.exists(p)]
-        if missing:
-            raise FileNotFoundError(f"pre-built reduced matrices missing: {missing}")
-        M = {k: read_matrix(p) for k, p in paths.items()}
         inv_path = os.path.join(reduced_path, _INVERSE)
-        if os.path.exists(inv_path):
+        missing = [p for p in [*paths.values(), inv_path] if not os.path.exists(p)]
+        if missing:
+            raise FileNotFoundError(f"pre-built reduced matrices missing (build the products first): {missing}")
+        M = {k: read_matrix(p) for k, p in paths.items()}
+        try:
             inverse_product = read_matrix(inv_path)
-        else:
+        except ValueError:
             inverse_product = M["ptov"] @ np.linalg.inv(M["A"]) @ M["vtod"]
             write_matrix(inv_path, inverse_product)
         return cls(M["ptof"], M["damping"], inverse_product)

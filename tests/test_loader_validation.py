@@ -69,3 +69,36 @@ def test_more_columns_than_supported_are_rejected(tmp_path):
     with pytest.raises(CfsError) as e:
         CompressedBasis.from_files(H.write_files(bufs, str(tmp_path / "wide.")))
     assert "CFS_ERR_FORMAT" in str(e.value) and "exceeds the supported maximum 256" in str(e.value), str(e.value)
+
+
+def test_header_sizes_that_would_wrap_are_rejected():
+    """A header whose numCols * numBlocks * 24 wraps a 64-bit size must not
+    pass the size check (it used to make a 72-byte component look whole and
+    send the segment loop to wild addresses); numCols is checked before any
+    size arithmetic, and numBlocks against the padded dims in 64 bits."""
+    import ctypes
+    import struct
+    from cfs_plume_amd import _lib
+    lib = _lib.lib()
+
+    def try_open(header, payload=b"\0" * 32):
+        img = struct.pack("<10i", *header) + payload
+        buf = ctypes.create_string_buffer(img, len(img))
+        h = ctypes.c_void_p()
+        st = lib.cfs_basis_open_memory(buf, len(img), buf, len(img), buf, len(img), ctypes.byref(h))
+        return st, lib.cfs_last_error().decode(), h.value
+
+    wrap = [1, 32, 8, 16, 1684887088, 8, 16, 1684887088, 1824726041, 421221772]
+    st, msg, h = try_open(wrap)
+    assert st == -3 and not h, (st, msg)
+    assert "numCols 1824726041 exceeds the supported maximum" in msg
+    # the same header with a plausible column count still fails on size, not on memory
+    st, msg, h = try_open(wrap[:8] + [2, 421221772])
+    assert st == -3 and "truncated" in msg and not h, (st, msg)
+    # padded dims whose block count overflows 64 bits unless computed in steps
+    big = 2147483640
+    st, msg, h = try_open([1, 32, big, big, big, big, big, big, 1, 8])
+    assert st == -3 and "numBlocks" in msg and not h, (st, msg)
+    # numBlocks <= 0
+    st, msg, h = try_open([1, 32, 8, 8, 8, 8, 8, 8, 1, 0])
+    assert st == -3 and "numBlocks" in msg and not h, (st, msg)

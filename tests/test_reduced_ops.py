@@ -20,11 +20,21 @@ def _write_set(d, r, rng, A=None):
     return mats
 
 
-def test_missing_inverse_product_is_built_and_written(tmp_path):
+def test_missing_inverse_product_is_refused(tmp_path):
+    """initOutOfCore (:84-139) needs all six files, inverseProduct included."""
+    _write_set(tmp_path, 4, np.random.default_rng(3))
+    with pytest.raises(FileNotFoundError) as e:
+        ReducedOperators.load(str(tmp_path))
+    assert "inverseProduct.matrix" in str(e.value)
+
+
+def test_unreadable_inverse_product_is_rebuilt_and_written(tmp_path):
     r = 5
     rng = np.random.default_rng(1)
     # A = 2 I: A^-1 = I / 2, so inverseProduct = ptov vtod / 2
     mats = _write_set(tmp_path, r, rng, A=2.0 * np.eye(r))
+    # present but short: EIGEN::read returns false, and the product is rebuilt
+    (tmp_path / "inverseProduct.matrix").write_bytes(b"\x05\x00")
     ops = ReducedOperators.load(str(tmp_path))
     want = 0.5 * mats["projected.ptov.matrix"] @ mats["projected.vtod.matrix"]
     assert np.allclose(ops.inverse_product, want, rtol=0, atol=1e-13)
