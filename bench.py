@@ -152,6 +152,25 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def time_launches(fn, steps, warmup, stream):
+    """`warmup` untimed calls, then `steps` calls with a HIP event recorded on
+    `stream` after each: (mean ms per call over the whole region, per-call ms)."""
+    from cfs_plume_amd.hip import Event, synchronize
+    for _ in range(warmup):
+        fn()
+    stream.synchronize()
+    ev = [Event() for _ in range(steps + 1)]
+    synchronize()
+    ev[0].record(stream)
+    for i in range(steps):
+        fn()
+        ev[i + 1].record(stream)
+    stream.synchronize()
+    synchronize()
+    per = np.array([ev[i].elapsed_ms(ev[i + 1]) for i in range(steps)])
+    return ev[0].elapsed_ms(ev[-1]) / steps, per
+
+
 def run_closure(args):
     """Learned closure on the 48x64x48, r = 52 basis: exact reduced-plume
     trajectories (U^T A(U q) on the full grid, the quantity the reference's
@@ -417,39 +436,60 @@ def main():
     fo = DeviceArray.zeros((len(v),), np.float32)
     stream = Stream()
 
-    def run_once():
-        if args.mode == "step":
-            C.step(basis, basis, dv, qo, dq, fo, stream)
-        elif args.mode == "project":
-            basis.project(dv, qo, stream)
-        else:
-            basis.unproject(dq, fo, stream)
+    fused = lambda: C.step(basis, basis, dv, qo, dq, fo, stream)  # noqa: E731
+    project = lambda: basis.project(dv, qo, stream)  # noqa: E731
+    unproject = lambda: basis.unproject(dq, fo, stream)  # noqa: E731
+    run_once = {"step": fused, "project": project, "unproject": unproject}[args.mode]
 
-    for _ in range(args.warmup):
-        run_once()
-    stream.synchronize()
-    basis.raise_on_stream_errors()
+    # Spatial decode of every column (DecodeVectorField for c = 0..r-1): the
+    # inverse DCTs run on MFMA (BF16 hi + lo products, FP32 accumulation).
+    # Timed before the step; its per-column error is checked further down.
+    spatial = None
+    cols = None
+    if not args.no_spatial:
+        cols = DeviceArray((r, 3 * basis.peeled_cells), np.float32)
+        basis.decode_columns(0, r, cols, stream)
+        stream.synchronize()
+        e0, e1 = Event(), Event()
+        e0.record(stream)
+        for _ in range(3):
+            basis.decode_columns(0, r, cols, stream)
+        e1.record(stream)
+        stream.synchronize()
+        basis.raise_on_stream_errors()
+        ms_dec = e0.elapsed_ms(e1) / 3
+        out_bytes = r * 3 * basis.peeled_cells * 4
+        in_bytes = 4 * sum(tokens) + 24 * 3 * r * basis.nB
+        spatial = {
+            "columns": r, "ms": round(ms_dec, 4), "output_GB": round(out_bytes / 1e9, 3),
+            "GB_per_s": round((out_bytes + in_bytes) / (ms_dec * 1e-3) / 1e9, 1),
+            "inverse_dct": "MFMA bf16 hi+lo split products, fp32 accumulation",
+        }
+        log(f"[bench] spatial decode of {r} columns: {ms_dec:.3f} ms ({spatial['GB_per_s']} GB/s read+written)")
 
-    # per-launch HIP events on the step's stream
-    ev = [Event() for _ in range(args.steps + 1)]
-    synchronize()
+    # The two halves of the step on their own (an integrator's step runs
+    # project with U_preadvect and unproject with U_final, two passes).
+    halves = {}
+    if args.mode == "step":
+        for name, fn in (("project", project), ("unproject", unproject)):
+            ms_h, per_h = time_launches(fn, args.steps, args.warmup, stream)
+            halves[f"{name}_ms"] = round(ms_h, 5)
+            log(f"[bench] {name} alone: {ms_h:.4f} ms/launch (median {np.median(per_h):.4f})")
+        halves["sum_ms"] = round(halves["project_ms"] + halves["unproject_ms"], 5)
+        basis.raise_on_stream_errors()
+
+    # the timed region: W untimed steps, then K steps bracketed by HIP events
+    # on the step's stream (one event per launch)
     t_wall = time.perf_counter()
-    ev[0].record(stream)
-    for i in range(args.steps):
-        run_once()
-        ev[i + 1].record(stream)
-    stream.synchronize()
-    synchronize()
+    ms_step, per = time_launches(run_once, args.steps, args.warmup, stream)
     t_wall = time.perf_counter() - t_wall
-    per = np.array([ev[i].elapsed_ms(ev[i + 1]) for i in range(args.steps)])
-    ms_total = ev[0].elapsed_ms(ev[-1])
-    ms_step = ms_total / args.steps
     basis.raise_on_stream_errors()
     step_bytes = basis.step_bytes()
     kernel_ms = float(per.mean())
     achieved = step_bytes / (kernel_ms * 1e-3) / 1e9
-    log(f"[bench] mode={args.mode} {args.steps} steps: {ms_step:.4f} ms/step (events), wall {t_wall / args.steps * 1e3:.4f} ms/step, "
-        f"median launch {np.median(per):.4f} ms, B_step={step_bytes / 1e9:.3f} GB -> {achieved:.0f} GB/s")
+    log(f"[bench] mode={args.mode} {args.steps} steps: {ms_step:.4f} ms/step (events), "
+        f"launches first {per[0]:.4f} min {per.min():.4f} median {np.median(per):.4f} max {per.max():.4f} ms, "
+        f"B_step={step_bytes / 1e9:.3f} GB -> {achieved:.0f} GB/s")
 
     q_gpu = qo.numpy()
     f_gpu = fo.numpy().astype(np.float64)
@@ -483,6 +523,11 @@ def main():
             "traffic": traffic_from_profiles(args.workload),
         },
     }
+    result["launch_ms"] = {"first": round(float(per[0]), 5), "min": round(float(per.min()), 5),
+                           "median": round(float(np.median(per)), 5), "max": round(float(per.max()), 5),
+                           "mean": round(kernel_ms, 5)}
+    if halves:
+        result["halves"] = halves
     mops = mfma_mops_from_profiles(args.workload)
     if mops is not None and args.mode == "step":
         # the inverse DCT is the only MFMA work (BF16 hi + lo products)
@@ -491,30 +536,8 @@ def main():
                           "peak_tflops": 2500.0, "frac": round(tflops / 2500.0, 5),
                           "counter": "SQ_INSTS_VALU_MFMA_MOPS_BF16 x 512 from the committed PMC profile"}
 
-    # Spatial decode of every column (DecodeVectorField for c = 0..r-1): the
-    # inverse DCTs run on MFMA (BF16 hi + lo products, FP32 accumulation).
-    spatial = None
-    if not args.no_spatial:
-        cols = DeviceArray((r, 3 * basis.peeled_cells), np.float32)
-        basis.decode_columns(0, r, cols, stream)
-        stream.synchronize()
-        e0, e1 = Event(), Event()
-        e0.record(stream)
-        for _ in range(3):
-            basis.decode_columns(0, r, cols, stream)
-        e1.record(stream)
-        stream.synchronize()
-        basis.raise_on_stream_errors()
-        ms_dec = e0.elapsed_ms(e1) / 3
-        out_bytes = r * 3 * basis.peeled_cells * 4
-        in_bytes = 4 * sum(tokens) + 24 * 3 * r * basis.nB
-        spatial = {
-            "columns": r, "ms": round(ms_dec, 4), "output_GB": round(out_bytes / 1e9, 3),
-            "GB_per_s": round((out_bytes + in_bytes) / (ms_dec * 1e-3) / 1e9, 1),
-            "inverse_dct": "MFMA bf16 hi+lo split products, fp32 accumulation",
-        }
+    if spatial is not None:
         result["spatial_decode"] = spatial
-        log(f"[bench] spatial decode of {r} columns: {ms_dec:.3f} ms ({spatial['GB_per_s']} GB/s read+written)")
 
     if not args.no_cpu:
         ns = min(args.cpu_sample_cols, r)
@@ -561,20 +584,28 @@ def main():
             "field_rel_l2": float(np.linalg.norm(f_gpu_s - f_s) / np.linalg.norm(f_s)),
             "columns": ns,
         }
+        # per-entry error of q relative to its largest entry
+        result["parity"]["q_max_entry_rel"] = float(np.max(np.abs(q_gpu_s - q_s)) / np.max(np.abs(q_s)))
         if spatial is not None:
-            # per-column error of the spatial decode against the FP64 oracle (sampled columns)
+            # every column of the spatial decode against the FP64 oracle's
+            # DecodeVectorField (threaded, identical results)
             from oracle import oracle as O
             from cfs_plume_amd.hip import check, hip
             ob = O.Basis(buffers=bufs)
             errs = []
-            for c in sorted({0, r // 3, (2 * r) // 3, r - 1}):
-                got = np.empty(3 * basis.peeled_cells, np.float32)
-                check(hip().hipMemcpy(got.ctypes.data, cols.ptr + c * got.nbytes, got.nbytes, 2), "hipMemcpy D2H")
-                want = ob.decode_vector_field(c)
-                errs.append(float(np.linalg.norm(got - want) / np.linalg.norm(want)))
+            n = 3 * basis.peeled_cells
+            for c0 in range(0, r, 16):
+                c1 = min(r, c0 + 16)
+                got = np.empty((c1 - c0, n), np.float32)
+                check(hip().hipMemcpy(got.ctypes.data, cols.ptr + c0 * n * 4, got.nbytes, 2), "hipMemcpy D2H")
+                want = ob.decode_vector_fields_mt(c0, c1, threads)
+                errs += [float(np.linalg.norm(got[i] - want[i]) / np.linalg.norm(want[i])) for i in range(c1 - c0)]
             ob.close()
             spatial["column_rel_l2_max"] = max(errs)
-            spatial["columns_checked"] = 4
+            spatial["column_rel_l2_mean"] = float(np.mean(errs))
+            spatial["columns_checked"] = len(errs)
+            log(f"[bench] decoded columns vs FP64 oracle: max {max(errs):.2e} mean {np.mean(errs):.2e} "
+                f"over {len(errs)} columns")
     if args.copy_calibration_mb:
         n = args.copy_calibration_mb << 20
         src = DeviceArray.zeros((n,), np.uint8)

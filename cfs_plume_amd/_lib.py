@@ -30,6 +30,7 @@ SIGNATURES = {
     "cfs_unproject": (_i, [_vp, _vp, _vp, _vp]),
     "cfs_step": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "cfs_decode_columns": (_i, [_vp, _i, _i, _vp, _vp]),
+    "cfs_untransform_svd": (_i, [_vp, _i, ctypes.c_int64, _vp, _vp]),
     "cfs_decode_block_ints": (_i, [_vp, _i, _i, _i, _i, _vp, _vp]),
     "cfs_decode_block_coeffs": (_i, [_vp, _i, _i, _i, _i, _vp, _vp]),
     "cfs_get_submatrix": (_i, [_vp, _i, _vp, _vp]),
@@ -60,6 +61,8 @@ SIGNATURES = {
     "cfs_closure_train_step": (_i, [_vp, _vp, _vp, _vp, _i, ctypes.c_float, _vp, _vp]),
     "cfs_closure_step": (_i, [_vp, _vp, _vp, _i, ctypes.c_double, _vp, _vp]),
     "cfs_encode_matrix": (_i, [_vp, _i, _i, _i, _i, _i, ctypes.c_double, _i, _i, ctypes.POINTER(_vp)]),
+    "cfs_encode_block": (_i, [_vp, _i, ctypes.c_double, _i, ctypes.POINTER(ctypes.c_double),
+                              ctypes.POINTER(ctypes.c_double), _vp, ctypes.POINTER(_i)]),
     "cfs_encoded_buffer": (_i, [_vp, _i, ctypes.POINTER(_vp), ctypes.POINTER(_sz)]),
     "cfs_encoded_write": (_i, [_vp, _cp]),
     "cfs_encoded_free": (None, [_vp]),

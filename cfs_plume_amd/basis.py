@@ -77,6 +77,20 @@ class EncodedBasis:
             pass
 
 
+def encode_block(F: np.ndarray, n_bits: int = 32, percent: float = 0.9995, max_iterations: int = 50):
+    """EncodeBlock (with PreprocessBlock and TuneGammaQuantized) of one
+    forward-DCT block F (512 FP64, u+8v+64w order): returns (s, gamma,
+    RLE tokens of the zigzag-ordered ints)."""
+    B = np.ascontiguousarray(F, dtype=np.float64).reshape(-1)
+    if B.size != 512:
+        raise ValueError("a block holds 512 coefficients")
+    s, g, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
+    tok = np.zeros(768, np.int32)
+    _lib.check(_lib.lib().cfs_encode_block(B.ctypes.data, n_bits, float(percent), max_iterations, ctypes.byref(s),
+                                           ctypes.byref(g), tok.ctypes.data, ctypes.byref(n)), "EncodeBlock")
+    return s.value, g.value, tok[:n.value].copy()
+
+
 def encode_matrix(U: np.ndarray, dims, n_bits: int = 32, percent: float = 0.9995, max_iterations: int = 50,
                   threads: int = 0) -> EncodedBasis:
     """CompressAndWriteMatrixComponents on U given as (r, 3*X*Y*Z) FP64 rows
@@ -157,6 +171,17 @@ class CompressedBasis:
             out = DeviceArray((ncols, 3 * self.peeled_cells), np.float32)
         _lib.check(_lib.lib().cfs_decode_columns(self._h, col0, ncols, out.ptr, _sptr(stream)), "DecodeVectorField")
         return out
+
+    def untransform_svd(self, columns: DeviceArray, V: DeviceArray, stream: Optional[Stream] = None) -> DeviceArray:
+        """UntransformVectorFieldSVD on decoded columns (ncols x 3*cells FP32,
+        in place) with one row-major 3 x 3 FP64 matrix per column in V."""
+        ncols = columns.shape[0] if len(columns.shape) > 1 else 1
+        cells = columns.nbytes // columns.dtype.itemsize // (3 * ncols)
+        if V.nbytes != 72 * ncols or V.dtype != np.float64:
+            raise ValueError("V must hold one FP64 3 x 3 matrix per column")
+        _lib.check(_lib.lib().cfs_untransform_svd(columns.ptr, ncols, cells, V.ptr, _sptr(stream)),
+                   "UntransformVectorFieldSVD")
+        return columns
 
     def decode_block_ints(self, comp: int, col: int, block0: int = 0, nblocks: Optional[int] = None,
                           stream: Optional[Stream] = None) -> np.ndarray:

@@ -37,6 +37,7 @@ void fill_tables(void* host_tab, const float* dct, const uint16_t* zig, const ui
 void fill_mfma_frags(void* host_tab);
 hipError_t launch_decode(const DevBasis& B, int mode, const DecodeArgs& a, const float* d_damp, const void* d_tab,
                          hipStream_t s);
+hipError_t launch_untransform_svd(float* cols, long cells, int ncols, const double* V, hipStream_t s);
 }  // namespace cfs
 
 using namespace cfs;
@@ -699,6 +700,15 @@ int cfs_get_submatrix(cfs_basis* B, int start_row, double* out, void* stream) {
   const int nbx = B->Px / BS, nby = B->Py / BS;
   const int blk = x / BS + (y / BS) * nbx + (z / BS) * nbx * nby;
   return decode_common(B, 3, 0, 3, 0, B->r, blk, 1, point, out, stream);
+}
+
+int cfs_untransform_svd(float* columns, int ncols, int64_t cells, const double* V, void* stream) {
+  if (ncols < 0 || cells < 0) return fail(CFS_ERR_ARG, "negative size");
+  if (ncols == 0 || cells == 0) return CFS_OK;
+  if (!columns || !V) return fail(CFS_ERR_ARG, "null argument");
+  if (ncols > 65535) return fail(CFS_ERR_ARG, "more than 65535 columns in one call");
+  hipError_t e = launch_untransform_svd(columns, (long)cells, ncols, V, (hipStream_t)stream);
+  return e == hipSuccess ? CFS_OK : hip_fail(e, "untransform launch");
 }
 
 int cfs_check_stream_errors(cfs_basis* B, uint32_t* bits) {

@@ -365,6 +365,33 @@ spatial_kernel(DevBasis B, DecodeArgs a, const float* __restrict__ g_damp, const
   if (!complete && lane == 0) atomicOr(B.err, (uint32_t)ERR_INCOMPLETE);
 }
 
+// UntransformVectorFieldSVD (reference COMPRESSION.cpp:713-727) on decoded
+// columns: every cell's xyz triple of column c becomes V_c u' (V_c row-major
+// 3 x 3, FP64; the reference multiplies the interleaved column by the block
+// diagonal of V_c, SparseBlockDiagonal :566-598).  One thread per cell of one
+// column; the FP32 triple is read once and written once.
+__global__ void __launch_bounds__(256) untransform_svd_kernel(float* __restrict__ cols, long cells, int ncols,
+                                                              const double* __restrict__ V) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = (int)blockIdx.y;
+  if (i >= cells || c >= ncols) return;
+  const double* m = V + 9 * (size_t)c;
+  float* p = cols + (size_t)c * 3 * (size_t)cells + 3 * (size_t)i;
+  const double a = p[0], b = p[1], d = p[2];
+  p[0] = (float)(m[0] * a + m[1] * b + m[2] * d);
+  p[1] = (float)(m[3] * a + m[4] * b + m[5] * d);
+  p[2] = (float)(m[6] * a + m[7] * b + m[8] * d);
+}
+
+hipError_t launch_untransform_svd(float* cols, long cells, int ncols, const double* V, hipStream_t s) {
+  if (cells <= 0 || ncols <= 0) return hipSuccess;
+  const long blocks = (cells + 255) / 256;
+  if (blocks > 0x7fffffffL || ncols > 65535) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(untransform_svd_kernel, dim3((unsigned)blocks, (unsigned)ncols), dim3(256), 0, s, cols, cells,
+                     ncols, V);
+  return hipGetLastError();
+}
+
 // Host: the MFMA operands of the inverse DCT in lane order (FP64 closed form
 // split into BF16 hi + lo, round to nearest even).
 static uint16_t host_bf16_bits(float x) {

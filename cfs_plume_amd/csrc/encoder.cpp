@@ -244,6 +244,19 @@ int cfs_encode_matrix(const double* U, int X, int Y, int Z, int r, int n_bits, d
   return CFS_OK;
 }
 
+int cfs_encode_block(const double* F, int n_bits, double percent, int max_iterations, double* s, double* gamma,
+                     int32_t* tokens, int* ntokens) {
+  if (!F || !s || !gamma || !tokens || !ntokens || n_bits < 1 || n_bits > 32 || max_iterations < 0)
+    return CFS_ERR_ARG;
+  double B[BC];
+  std::memcpy(B, F, sizeof(B));
+  std::vector<int32_t> t;
+  encode_block(B, n_bits, percent, max_iterations, consts(), s, gamma, &t);
+  std::memcpy(tokens, t.data(), t.size() * 4);
+  *ntokens = (int)t.size();
+  return CFS_OK;
+}
+
 int cfs_encoded_buffer(const cfs_encoded* enc, int comp, const void** data, size_t* size) {
   if (!enc || comp < 0 || comp > 2 || !data || !size) return CFS_ERR_ARG;
   *data = enc->file[comp].data();

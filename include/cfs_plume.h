@@ -96,6 +96,14 @@ int cfs_step(cfs_basis* pre, cfs_basis* fin, const float* v_in, double* q_out, c
  * and unpadded, interleaved xyz (the layout of one column of U). */
 int cfs_decode_columns(cfs_basis* basis, int col0, int ncols, float* out, void* stream);
 
+/* UntransformVectorFieldSVD (COMPRESSION.cpp:713-727) applied to ncols
+ * decoded columns in place: columns holds ncols x 3*cells FP32 values in the
+ * layout of cfs_decode_columns, V (device) holds ncols row-major 3 x 3 FP64
+ * matrices; every cell's xyz triple u' of column c becomes V_c u'.  The
+ * reference's live decode has the transform switched off (DecodeVectorField
+ * :2667-2668, encoder :2312), so cfs_decode_columns does not call it. */
+int cfs_untransform_svd(float* columns, int ncols, int64_t cells, const double* V, void* stream);
+
 /* RunLengthDecodeBinary + ZigzagUnflatten (COMPRESSION.cpp:1198-1283,
  * 1427-1445) for blocks [block0, block0+nblocks) of (comp, col): out holds
  * nblocks x 512 int32 in u+8v+64w order.  Bit-exact check hook. */
@@ -211,6 +219,13 @@ int cfs_closure_step(cfs_closure* c, const double* q_in, double* q_out, int n, d
  * zigzag, RLE.  `threads` <= 0 uses all hardware threads. */
 int cfs_encode_matrix(const double* U, int X, int Y, int Z, int r, int n_bits, double percent, int max_iterations,
                       int threads, cfs_encoded** out);
+/* One block of the same encoder: PreprocessBlock + TuneGammaQuantized +
+ * EncodeBlock + ZigzagFlatten + RunLengthEncodeBinary (COMPRESSION.cpp:733-759,
+ * 841-949, 1402-1421, 1112-1192) on F, a forward-DCT block of 512 FP64
+ * coefficients in u+8v+64w order.  Writes s, gamma and *ntokens (<= 768)
+ * tokens to `tokens`. */
+int cfs_encode_block(const double* F, int n_bits, double percent, int max_iterations, double* s, double* gamma,
+                     int32_t* tokens, int* ntokens);
 /* The in-memory image of component file `comp` (0, 1, 2). */
 int cfs_encoded_buffer(const cfs_encoded* enc, int comp, const void** data, size_t* size);
 /* Writes <prefix>0, <prefix>1, <prefix>2 (the reference names U.<basis>.component0..2). */

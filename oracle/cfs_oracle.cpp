@@ -20,6 +20,7 @@
 //
 // Build: oracle/Makefile (g++ -O2 -ffp-contract=off, single-threaded).
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <functional>
 #include <cstdint>
@@ -614,6 +615,27 @@ int orc_decode_vector_field(void* h, int col, double* out) {
     for (size_t i = 0; i < N; i++) out[3 * i + k] = F[i];
   }
   return ORC_OK;
+}
+
+// orc_decode_vector_field for columns [col0, col1) on `threads` host threads
+// (one column per task; each column's arithmetic is unchanged), into
+// out[(col - col0) * 3*X*Y*Z ...].
+int orc_decode_vector_fields_mt(void* h, int col0, int col1, int threads, double* out) {
+  Basis* B = (Basis*)h;
+  const Component& c0 = B->comp[0];
+  if (col0 < 0 || col1 > c0.r || col0 > col1) return ORC_ERR_RANGE;
+  const size_t n = (size_t)3 * c0.X * c0.Y * c0.Z;
+  std::atomic<int> next(col0), status(ORC_OK);
+  std::vector<std::thread> th;
+  for (int t = 0; t < std::max(1, threads); t++)
+    th.emplace_back([&] {
+      for (int c = next++; c < col1; c = next++) {
+        const int st = orc_decode_vector_field(h, c, out + (size_t)(c - col0) * n);
+        if (st) status = st;
+      }
+    });
+  for (auto& x : th) x.join();
+  return status;
 }
 
 // PeeledCompressedProjectTransformNoSVD COMPRESSION.cpp:2978-3016 for columns

@@ -57,6 +57,7 @@ def _load():
         "orc_decode_block_ints": (c_int, [c_void_p, c_int, c_int, c_int, _i32p, ctypes.POINTER(c_int)]),
         "orc_decode_block_coeffs": (c_int, [c_void_p, c_int, c_int, c_int, c_int, _f64p]),
         "orc_decode_vector_field": (c_int, [c_void_p, c_int, _f64p]),
+        "orc_decode_vector_fields_mt": (c_int, [c_void_p, c_int, c_int, c_int, _f64p]),
         "orc_project": (c_int, [c_void_p, _f64p, _f64p, c_int, c_int]),
         "orc_project_mt": (c_int, [c_void_p, _f64p, _f64p, c_int, c_int, c_int]),
         "orc_unproject_mt": (c_int, [c_void_p, _f64p, _f64p, c_int, c_int, c_int]),
@@ -273,6 +274,14 @@ class Basis:
         X, Y, Z = self.dims
         out = np.zeros(3 * X * Y * Z, np.float64)
         _check(lib().orc_decode_vector_field(self._h, col, out), "DecodeVectorField")
+        return out
+
+    def decode_vector_fields_mt(self, col0, col1, threads):
+        """decode_vector_field for columns [col0, col1) on `threads` threads
+        (identical results), as an array [col1 - col0][3*X*Y*Z]."""
+        X, Y, Z = self.dims
+        out = np.zeros((col1 - col0, 3 * X * Y * Z), np.float64)
+        _check(lib().orc_decode_vector_fields_mt(self._h, col0, col1, threads, out), "DecodeVectorField (threads)")
         return out
 
     def project(self, V_full, col0=0, col1=None):

@@ -327,3 +327,56 @@ def test_fused_step_replays_in_a_hip_graph(smooth):
         s.synchronize()
         assert np.array_equal(q_e.numpy(), eager[0])
     assert gb.check_stream_errors() == 0
+
+
+def test_step_with_two_bases(smooth, tmp_path_factory):
+    """stepPlume projects with U_preadvect and unprojects with U_final
+    (SUBSPACE_FLUID_3D_COMPRESSED_EIGEN.cpp:477,490): cfs_step(pre, fin)
+    gives q_out = pre^T v and field = fin q_in, each against its own basis."""
+    ob_pre, gb_pre = smooth
+    dims = ob_pre.dims
+    U2 = synth.synthetic_basis(dims, 5, seed=31)
+    print(f"[case] second basis dims={dims} r=5 (the unproject basis)", flush=True)
+    ob_fin, gb_fin = _pair(encode_matrix(U2, dims).buffers(), tmp_path_factory, "final")
+    v = synth.gaussian_field(dims, 46)
+    q = synth.gaussian_q(gb_fin.r, 47)
+    dv = DeviceArray.from_numpy(v.astype(np.float32))
+    dq = DeviceArray.from_numpy(q)
+    qo = DeviceArray.zeros((gb_pre.r,), np.float64)
+    fo = DeviceArray.zeros((len(v),), np.float32)
+    step(gb_pre, gb_fin, dv, qo, dq, fo)
+    want_q = ob_pre.project(v.astype(np.float32).astype(np.float64))
+    want_f = ob_fin.unproject(q, np.zeros(len(v)))
+    assert rel(qo.numpy(), want_q) <= TOL
+    assert rel(fo.numpy(), want_f) <= TOL
+    # the two halves used different bases: swapping them changes both results
+    assert rel(ob_fin.project(v.astype(np.float32).astype(np.float64)), want_q[:gb_fin.r]) > 0.1
+    assert gb_pre.check_stream_errors() == 0 and gb_fin.check_stream_errors() == 0
+
+
+def test_decode_columns_empty_range(smooth):
+    """An empty column range decodes nothing and succeeds, as the block
+    decoders do for an empty block range."""
+    import ctypes
+    from cfs_plume_amd import _lib
+    ob, gb = smooth
+    out = DeviceArray.from_numpy(np.full(16, 3.5, np.float32))
+    assert _lib.lib().cfs_decode_columns(gb._h, 0, 0, ctypes.c_void_p(out.ptr), None) == 0
+    assert _lib.lib().cfs_decode_columns(gb._h, ob.r, 0, ctypes.c_void_p(out.ptr), None) == 0
+    assert np.all(out.numpy() == 3.5)
+
+
+def test_untransform_svd_matches_oracle(ragged):
+    """UntransformVectorFieldSVD (COMPRESSION.cpp:713-727) on decoded columns:
+    each column's cells multiplied by its own 3 x 3 V, against the oracle's
+    restatement applied to the oracle's FP64 decode."""
+    ob, gb = ragged
+    ncols = 7
+    rng = np.random.default_rng(5)
+    Vs = np.stack([np.linalg.qr(rng.standard_normal((3, 3)))[0] for _ in range(ncols)])
+    cols = gb.decode_columns(3, ncols)
+    gb.untransform_svd(cols, DeviceArray.from_numpy(Vs.reshape(-1)))
+    got = cols.numpy()
+    for i in range(ncols):
+        want = O.untransform_svd(Vs[i], ob.decode_vector_field(3 + i))
+        assert rel(got[i], want) <= SPATIAL_TOL, (i, rel(got[i], want))
