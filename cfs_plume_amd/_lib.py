@@ -45,6 +45,7 @@ SIGNATURES = {
     "cfs_batch_graph_launch": (_i, [_vp, _vp]),
     "cfs_batch_destroy": (_i, [_vp]),
     "cfs_reduced_advect": (_i, [_vp, _vp, _vp, _vp, _vp, ctypes.c_float, ctypes.c_double, _vp, _vp]),
+    "cfs_reduced_advect_noalloc": (_i, [_vp, _vp, _vp, ctypes.c_float, ctypes.c_double, _vp, _vp]),
     "cfs_advect_field": (_i, [_vp, _vp, _i, _i, _i, ctypes.c_float, _vp]),
     "cfs_advect_scalar": (_i, [_vp, _vp, _vp, _i, _i, _i, ctypes.c_float, _vp]),
     "cfs_closure_create": (_i, [_i, _i, _i, ctypes.c_uint64, ctypes.POINTER(_vp)]),

@@ -164,6 +164,18 @@ int cfs_batch_destroy(cfs_batch* batch);
  * the reference its accuracy is measured against. */
 int cfs_reduced_advect(cfs_basis* U, const double* q_in, double* q_out, float* field_a, float* field_b, float dt0,
                        double alpha, const double* forcing, void* stream);
+/* cfs_reduced_advect without the two scratch fields.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, serving loops, harnesses that forbid hipMalloc).  The same
+ * launches as cfs_reduced_advect with field_a / field_b taken from one
+ * fixed device array that every GPU the library runs on holds until the
+ * process ends (2 x CFS_NOALLOC_FIELD_FLOATS floats = 32 MiB); its address is
+ * looked up on the first call on each device.  Calls nothing that allocates.
+ * Full grids with 3 (X+2)(Y+2)(Z+2) > CFS_NOALLOC_FIELD_FLOATS are refused
+ * with CFS_ERR_ARG.  Calls that overlap on one GPU race on the array. */
+#define CFS_NOALLOC_FIELD_FLOATS 4194304
+int cfs_reduced_advect_noalloc(cfs_basis* U, const double* q_in, double* q_out, float dt0, double alpha,
+                               const double* forcing, void* stream);
 /* VECTOR3_FIELD_3D::advect of a full-grid field by itself (dims >= 4). */
 int cfs_advect_field(const float* vel, float* out, int X, int Y, int Z, float dt0, void* stream);
 /* VECTOR3_FIELD_3D::advect of a full-grid scalar field (VECTOR3_FIELD_3D.cpp:2082-2160,
