@@ -46,6 +46,14 @@ SIGNATURES = {
     "cfs_batch_destroy": (_i, [_vp]),
     "cfs_reduced_advect": (_i, [_vp, _vp, _vp, _vp, _vp, ctypes.c_float, ctypes.c_double, _vp, _vp]),
     "cfs_reduced_advect_noalloc": (_i, [_vp, _vp, _vp, ctypes.c_float, ctypes.c_double, _vp, _vp]),
+    "cfs_cubature_create": (_i, [_i, _vp, _vp, _vp, ctypes.POINTER(_vp)]),
+    "cfs_cubature_open_file": (_i, [_cp, _vp, ctypes.POINTER(_vp)]),
+    "cfs_cubature_write_file": (_i, [_cp, _i, _vp, _vp]),
+    "cfs_cubature_get_info": (_i, [_vp, ctypes.POINTER(_i), ctypes.POINTER(_i)]),
+    "cfs_cubature_get_cells": (_i, [_vp, _vp, _vp]),
+    "cfs_cubature_advect": (_i, [_vp, _vp, _vp, _vp, _vp, ctypes.c_float, ctypes.c_double, _vp, _vp]),
+    "cfs_cubature_destroy": (_i, [_vp]),
+    "cfs_nnls": (_i, [_vp, _i, _i, _vp, _i, _vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_i)]),
     "cfs_advect_field": (_i, [_vp, _vp, _i, _i, _i, ctypes.c_float, _vp]),
     "cfs_advect_scalar": (_i, [_vp, _vp, _vp, _i, _i, _i, ctypes.c_float, _vp]),
     "cfs_closure_create": (_i, [_i, _i, _i, ctypes.c_uint64, ctypes.POINTER(_vp)]),

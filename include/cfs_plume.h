@@ -176,6 +176,42 @@ int cfs_reduced_advect(cfs_basis* U, const double* q_in, double* q_out, float* f
 #define CFS_NOALLOC_FIELD_FLOATS 4194304
 int cfs_reduced_advect_noalloc(cfs_basis* U, const double* q_in, double* q_out, float dt0, double alpha,
                                const double* forcing, void* stream);
+/* ---- cubature reduced advection --------------------------------------------
+ * The reference's per-step subspace advection, reducedAdvectCompressionFriendly
+ * (SUBSPACE_FLUID_3D_COMPRESSED_EIGEN.cpp:1781-1854, accumAdvectRequests
+ * :1859-1993): at K key cells x_i with weights w_i, backtrace by the velocity
+ * U_pre(x_i) q, interpolate U_pre q trilinearly there and sum
+ * q' = sum_i w_i U_after(x_i)^T a_i.  A cubature holds the key cells, weights
+ * and the after-matrices w_i U_after(x_i)^T (FP64), built on the device from
+ * the compressed basis `after` (readAdvectionCubature :1188-1327, which builds
+ * them from U.prediffuse). */
+typedef struct cfs_cubature cfs_cubature;
+/* cells: peeled cell indices x + X (y + Y z) of `after`'s grid. */
+int cfs_cubature_create(int K, const int32_t* cells, const double* weights, cfs_basis* after, cfs_cubature** out);
+/* The `cubature` file: int32 K, int32 cells[K], FP64 weights[K]
+ * (CUBATURE_GENERATOR_EIGEN::writeCubature, src/cubature/CUBATURE_GENERATOR_EIGEN.cpp:50-91;
+ * read by readAdvectionCubature :1195-1232).  A missing file is CFS_ERR_IO,
+ * a size that does not match K is CFS_ERR_FORMAT. */
+int cfs_cubature_open_file(const char* path, cfs_basis* after, cfs_cubature** out);
+int cfs_cubature_write_file(const char* path, int K, const int32_t* cells, const double* weights);
+int cfs_cubature_get_info(const cfs_cubature* c, int* K, int* r);
+int cfs_cubature_get_cells(const cfs_cubature* c, int32_t* cells, double* weights);
+/* q_out = alpha * (cubature advection of q_in) + forcing (FP64[r] or NULL);
+ * field: FP32 full-grid scratch (interior cells overwritten with U_pre q_in).
+ * Three launches (unproject, key cells, reduction), no allocation. */
+int cfs_cubature_advect(cfs_cubature* c, cfs_basis* pre, const double* q_in, double* q_out, float* field, float dt0,
+                        double alpha, const double* forcing, void* stream);
+int cfs_cubature_destroy(cfs_cubature* c);
+
+/* NNLS_SOLVER::solve (src/linearalgebra/NNLS.h:156-208, Lawson-Hanson) as the
+ * cubature generator calls it (CUBATURE_GENERATOR_EIGEN.cpp:654-663):
+ * x >= 0 minimising |A x - b|, A column-major m x n (host), at most
+ * max_iter_factor * n inner iterations (the reference uses 5); *iterations
+ * above that number means the cap was reached (x is then the last iterate,
+ * still non-negative).  Host only. */
+int cfs_nnls(const double* A, int m, int n, const double* b, int max_iter_factor, double* x, double* rnorm,
+             int* iterations);
+
 /* VECTOR3_FIELD_3D::advect of a full-grid field by itself (dims >= 4). */
 int cfs_advect_field(const float* vel, float* out, int X, int Y, int Z, float dt0, void* stream);
 /* VECTOR3_FIELD_3D::advect of a full-grid scalar field (VECTOR3_FIELD_3D.cpp:2082-2160,
