@@ -69,6 +69,12 @@ SIGNATURES = {
     "cfs_closure_gradients": (_i, [_vp, _vp, _vp, _i, _vp, _vp, _vp]),
     "cfs_closure_train_step": (_i, [_vp, _vp, _vp, _vp, _i, ctypes.c_float, _vp, _vp]),
     "cfs_closure_step": (_i, [_vp, _vp, _vp, _i, ctypes.c_double, _vp, _vp]),
+    "cfs_plume_default_params": (None, [_i, _i, _i, _vp]),
+    "cfs_plume_create": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(_vp)]),
+    "cfs_plume_set_advection": (_i, [_vp, _i, _vp, _vp]),
+    "cfs_plume_add_smoke": (_i, [_vp, _vp, _vp, _vp]),
+    "cfs_plume_step": (_i, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "cfs_plume_destroy": (_i, [_vp]),
     "cfs_encode_matrix": (_i, [_vp, _i, _i, _i, _i, _i, ctypes.c_double, _i, _i, ctypes.POINTER(_vp)]),
     "cfs_encode_block": (_i, [_vp, _i, ctypes.c_double, _i, ctypes.POINTER(ctypes.c_double),
                               ctypes.POINTER(ctypes.c_double), _vp, ctypes.POINTER(_i)]),
@@ -76,6 +82,18 @@ SIGNATURES = {
     "cfs_encoded_write": (_i, [_vp, _cp]),
     "cfs_encoded_free": (None, [_vp]),
 }
+
+
+class PlumeParams(ctypes.Structure):
+    _fields_ = [
+        ("dt", ctypes.c_double),
+        ("dx", ctypes.c_double),
+        ("buoyancy", ctypes.c_double),
+        ("vorticity_eps", ctypes.c_double),
+        ("open_x", ctypes.c_int),
+        ("open_y", ctypes.c_int),
+        ("open_z", ctypes.c_int),
+    ]
 
 
 class CfsError(RuntimeError):

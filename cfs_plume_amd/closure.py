@@ -253,13 +253,57 @@ class ReducedPlume:
     sphere)."""
 
     def __init__(self, U: CompressedBasis, dt0: float = 1.0, alpha: float = 0.98, force: float = 0.05,
-                 radius: float = 4.0):
+                 radius: float = 4.0, cubature=None):
         self.U, self.dt0, self.alpha, self.force, self.radius = U, dt0, alpha, force, radius
         X, Y, Z = U.dims
         self.full = (X + 2, Y + 2, Z + 2)
         n = 3 * self.full[0] * self.full[1] * self.full[2]
         self.fa = DeviceArray((n,), np.float32)
         self.fb = DeviceArray((n,), np.float32)
+        # the plain reduced solver's advection: the cubature when one is set
+        # (reducedAdvectCompressionFriendly), else the exact U^T A(U q)
+        self.cubature = cubature
+
+    def advect(self, q_in, q_out, alpha: float, forcing=None, stream: Optional[Stream] = None):
+        """q_out = alpha * Adv(q_in) + forcing with the plain solver's advection."""
+        if self.cubature is not None:
+            self.cubature.advect(self.U, q_in, q_out, self.fa, self.dt0, alpha, forcing, stream)
+        else:
+            reduced_advect(self.U, q_in, q_out, self.fa, self.fb, self.dt0, alpha, forcing, stream)
+
+    def rollout(self, b: DeviceArray, runs: int, steps: int, stream: Optional[Stream] = None) -> np.ndarray:
+        """Like rollout_exact with the plain solver's advection (the cubature
+        when one is set)."""
+        r = self.U.r
+        traj = DeviceArray.zeros((runs * (steps + 1), r), np.float64)
+        for i in range(runs):
+            for t in range(steps):
+                self.advect((traj, i * (steps + 1) + t), (traj, i * (steps + 1) + t + 1), self.alpha, (b, i), stream)
+        return traj.numpy().reshape(runs, steps + 1, r)
+
+    def increments_at(self, states: np.ndarray, stream: Optional[Stream] = None) -> np.ndarray:
+        """Adv(q) - q for every row of `states` with the plain solver's advection."""
+        r = self.U.r
+        n = states.shape[0]
+        qin = DeviceArray.from_numpy(np.ascontiguousarray(states, np.float64))
+        qout = DeviceArray((n, r), np.float64)
+        for i in range(n):
+            self.advect((qin, i), (qout, i), 1.0, None, stream)
+        return qout.numpy() - states
+
+    def make_cubature(self, states: np.ndarray, candidates_per_pass: int = 5000, passes: int = 6,
+                      tolerance: float = 0.01, log=None):
+        """Fits a cubature (generateImportanceSampledCubature) to the exact
+        reduced advection at `states` ([S][r], the training samples) and sets
+        it as this plume's advection.  Returns the generator's result."""
+        from . import cubature as CB
+        rows = CB.dense_rows(self.U)
+        post = self.exact_increments(states) + states
+        res = CB.generate_cubature(rows, rows, self.U.dims, states, post, self.dt0,
+                                   candidates_per_pass=candidates_per_pass, passes=passes, tolerance=tolerance,
+                                   log=log)
+        self.cubature = CB.Cubature.from_cells(res.cells, res.weights, self.U)
+        return res
 
     @staticmethod
     def seeded_runs(n: int, seed: int) -> Sequence[PlumeRun]:
@@ -323,10 +367,11 @@ class ReducedPlume:
 
     def rollout_closure_checked(self, closure: Closure, b: DeviceArray, run: int, steps: int, check_every: int,
                                 tol: float = 0.01, stream: Optional[Stream] = None):
-        """One run with the closure; every `check_every` steps the exact step is
-        also taken from the same state and replaces the closure's result when
-        their relative difference exceeds `tol` (the fallback).  Returns
-        (trajectory [steps+1][r], number of fallbacks)."""
+        """One run with the closure; every `check_every` steps the plain
+        solver's step (the cubature when one is set) is also taken from the
+        same state and replaces the closure's result when their relative
+        difference exceeds `tol` (the fallback).  Returns (trajectory
+        [steps+1][r], number of fallbacks)."""
         r = self.U.r
         q = DeviceArray.zeros((r,), np.float64)
         qc = DeviceArray.zeros((r,), np.float64)
@@ -336,7 +381,7 @@ class ReducedPlume:
         for t in range(steps):
             closure.step(q, qc, 1, self.alpha, (b, run), stream)
             if check_every and (t + 1) % check_every == 0:
-                reduced_advect(self.U, q, qe, self.fa, self.fb, self.dt0, self.alpha, (b, run), stream)
+                self.advect(q, qe, self.alpha, (b, run), stream)
                 a, e = qc.numpy(), qe.numpy()
                 if np.linalg.norm(a - e) > tol * max(np.linalg.norm(e), 1e-300):
                     falls += 1
@@ -359,6 +404,50 @@ def increments(traj: np.ndarray, b: np.ndarray, alpha: float):
     q = traj[:, :-1]
     d = (traj[:, 1:] - b[:, None, :]) / alpha - q
     return q.reshape(-1, q.shape[-1]), d.reshape(-1, d.shape[-1])
+
+
+def sample_states(traj: np.ndarray, n: int) -> np.ndarray:
+    """n states spread evenly over the steps t >= 1 of all runs of `traj`
+    ([runs][steps+1][r]): training samples for a cubature fit."""
+    q = traj[:, 1:].reshape(-1, traj.shape[-1])
+    idx = np.linspace(0, len(q) - 1, n).round().astype(np.int64)
+    return np.ascontiguousarray(q[idx])
+
+
+def cubature_training_states(plume: "ReducedPlume", b: DeviceArray, runs: int, steps: int, samples: int,
+                             noise: float = 0.1, seed: int = 3) -> np.ndarray:
+    """States a plume's cubature is fitted to: samples / 2 states spread over
+    exact rollouts of `runs` runs, and the same states perturbed by `noise`
+    times |q| / sqrt(r)."""
+    traj = plume.rollout_exact(b, runs, steps)
+    st = sample_states(traj, samples // 2)
+    rng = np.random.default_rng(seed)
+    pert = st + noise * rng.normal(size=st.shape) * np.linalg.norm(st, axis=1, keepdims=True) / np.sqrt(st.shape[1])
+    return np.concatenate([st, pert])
+
+
+def train_closure(plume: "ReducedPlume", b: DeviceArray, runs: int, steps: int, epochs: int, noise: float = 0.1,
+                  seed: int = 5, max_batch: int = 1024, log=None):
+    """Trains a closure on trajectories of the plain reduced solver (the
+    plume's advection: the cubature when one is set): `runs` x `steps` pairs
+    (q_t, Adv(q_t) - q_t), plus the same at states perturbed by `noise` times
+    |q| / sqrt(r) around them.  Returns (closure, {samples, losses, seconds})."""
+    import time
+    r = plume.U.r
+    t0 = time.perf_counter()
+    traj = plume.rollout(b, runs, steps)
+    q, d = increments(traj, b.numpy(), plume.alpha)
+    if noise > 0:
+        rng = np.random.default_rng(11)
+        qp = q + noise * rng.normal(size=q.shape) * np.linalg.norm(q, axis=1, keepdims=True) / np.sqrt(r)
+        q, d = np.concatenate([q, qp]), np.concatenate([d, plume.increments_at(qp)])
+    t_data = time.perf_counter() - t0
+    cl = Closure(r, max_batch=max_batch, seed=seed)
+    t0 = time.perf_counter()
+    losses = cl.fit(q, d, epochs=epochs, batch=512, lr=2e-3, lr_final=1e-5, seed=7,
+                    log=(lambda ep, m: log(f"closure epoch {ep}: mse {m:.3e}") if ep % 20 == 0 else None)
+                    if log else None)
+    return cl, {"samples": int(q.shape[0]), "losses": losses, "data_s": t_data, "train_s": time.perf_counter() - t0}
 
 
 def fit_scales(q: np.ndarray, d: np.ndarray, floor: float = 1e-12):

@@ -17,8 +17,8 @@
 // block decoded once) and the key-cell kernel interpolates that field:
 //
 //   launch 1   cfs_unproject(U_pre, q) -> field (FP32, interior cells)
-//   launch 2   workgroup of 256 key cells: the backtrace and trilinear a_i
-//              in FP64 from the field, then each thread j = 0..r-1 sums
+//   launch 2   workgroup of 16 key cells x 64 outputs: the backtrace and
+//              trilinear a_i in FP64 from the field, then lane j sums
 //              After_i[j] . a_i over the workgroup's cells (FP64) -> partials
 //   launch 3   q'_j = alpha * sum of the partials (workgroup order, FP64) + f_j
 //
@@ -38,71 +38,87 @@
 
 namespace {
 
-constexpr int CUB_TPB = 256;
+// A workgroup of one wave takes CUB_CELLS key cells and 64 of the r outputs:
+// lanes 0..CUB_CELLS-1 compute the cells' interpolated velocities (FP64) into
+// LDS, then lane j sums After_i[j] . a_i over the cells with every load of
+// the (unrolled) loop in flight.  Many small workgroups keep enough loads in
+// flight: the after-matrices are small (K x r x 24 bytes) and the kernel is
+// bound by load latency, not bandwidth.
+constexpr int CUB_CELLS = 16;
+constexpr int CUB_TPB = 64;
 
 __global__ void __launch_bounds__(CUB_TPB)
 cubature_kernel(const float* __restrict__ field, int X, int Y, int Z, const int32_t* __restrict__ cells, int K,
                 const double* __restrict__ after, int r, double dt0, double* __restrict__ partials) {
-  __shared__ double adv[CUB_TPB][3];
+  __shared__ double adv[CUB_CELLS][3];
   const int t = threadIdx.x;
-  const int i = blockIdx.x * CUB_TPB + t;
-  double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-  if (i < K) {
-    // peeled cell -> full-grid coordinates
-    const int slab = X * Y;
-    const int idx = cells[i];
-    const int z = idx / slab + 1, y = (idx % slab) / X + 1, x = (idx % slab) % X + 1;
-    const int GX = X + 2, GY = Y + 2;
-    const int gslab = GX * GY;
-    const float* f = field + 3 * ((size_t)x + (size_t)GX * ((size_t)y + (size_t)GY * z));
-    double xt = x - dt0 * (double)f[0];
-    double yt = y - dt0 * (double)f[1];
-    double zt = z - dt0 * (double)f[2];
-    // full-grid resolution is peeled + 2
-    xt = xt < 1.5 ? 1.5 : xt;
-    xt = xt > GX - 2.5 ? GX - 2.5 : xt;
-    yt = yt < 1.5 ? 1.5 : yt;
-    yt = yt > GY - 2.5 ? GY - 2.5 : yt;
-    zt = zt < 1.5 ? 1.5 : zt;
-    zt = zt > (Z + 2) - 2.5 ? (Z + 2) - 2.5 : zt;
-    // peeled corner x0 = (int)xt - 1 is full-grid cell (int)xt
-    const int cx = (int)xt, cy = (int)yt, cz = (int)zt;
-    const double s1 = xt - cx, s0 = 1.0 - s1;
-    const double t1 = yt - cy, t0 = 1.0 - t1;
-    const double u1 = zt - cz, u0 = 1.0 - u1;
-    const float* c000 = field + 3 * ((size_t)cx + (size_t)GX * ((size_t)cy + (size_t)GY * cz));
-    const float* c010 = c000 + 3 * GX;
-    const float* c100 = c000 + 3;
-    const float* c110 = c010 + 3;
-    const float* c001 = c000 + 3 * (size_t)gslab;
-    const float* c011 = c010 + 3 * (size_t)gslab;
-    const float* c101 = c100 + 3 * (size_t)gslab;
-    const float* c111 = c110 + 3 * (size_t)gslab;
-    const double w000 = u0 * s0 * t0, w010 = u0 * s0 * t1, w100 = u0 * s1 * t0, w110 = u0 * s1 * t1;
-    const double w001 = u1 * s0 * t0, w011 = u1 * s0 * t1, w101 = u1 * s1 * t0, w111 = u1 * s1 * t1;
-    double a[3];
+  const int i0 = blockIdx.x * CUB_CELLS;
+  if (t < CUB_CELLS) {
+    const int i = i0 + t;
+    double a[3] = {0.0, 0.0, 0.0};
+    if (i < K) {
+      // peeled cell -> full-grid coordinates
+      const int slab = X * Y;
+      const int idx = cells[i];
+      const int z = idx / slab + 1, y = (idx % slab) / X + 1, x = (idx % slab) % X + 1;
+      const int GX = X + 2, GY = Y + 2;
+      const size_t gslab = (size_t)GX * GY;
+      const float* f = field + 3 * ((size_t)x + (size_t)GX * ((size_t)y + (size_t)GY * z));
+      double xt = x - dt0 * (double)f[0];
+      double yt = y - dt0 * (double)f[1];
+      double zt = z - dt0 * (double)f[2];
+      // clamp to [1.5, res - 2.5], res the full-grid resolution (peeled + 2)
+      xt = xt < 1.5 ? 1.5 : xt;
+      xt = xt > GX - 2.5 ? GX - 2.5 : xt;
+      yt = yt < 1.5 ? 1.5 : yt;
+      yt = yt > GY - 2.5 ? GY - 2.5 : yt;
+      zt = zt < 1.5 ? 1.5 : zt;
+      zt = zt > (Z + 2) - 2.5 ? (Z + 2) - 2.5 : zt;
+      // peeled corner x0 = (int)xt - 1 is full-grid cell (int)xt
+      const int cx = (int)xt, cy = (int)yt, cz = (int)zt;
+      const double s1 = xt - cx, s0 = 1.0 - s1;
+      const double t1 = yt - cy, t0 = 1.0 - t1;
+      const double u1 = zt - cz, u0 = 1.0 - u1;
+      const float* c000 = field + 3 * ((size_t)cx + (size_t)GX * ((size_t)cy + (size_t)GY * cz));
+      const float* c010 = c000 + 3 * GX;
+      const float* c100 = c000 + 3;
+      const float* c110 = c010 + 3;
+      const float* c001 = c000 + 3 * gslab;
+      const float* c011 = c010 + 3 * gslab;
+      const float* c101 = c100 + 3 * gslab;
+      const float* c111 = c110 + 3 * gslab;
+      const double w000 = u0 * s0 * t0, w010 = u0 * s0 * t1, w100 = u0 * s1 * t0, w110 = u0 * s1 * t1;
+      const double w001 = u1 * s0 * t0, w011 = u1 * s0 * t1, w101 = u1 * s1 * t0, w111 = u1 * s1 * t1;
 #pragma unroll
-    for (int c = 0; c < 3; c++)
-      a[c] = w000 * c000[c] + w010 * c010[c] + w100 * c100[c] + w110 * c110[c] + w001 * c001[c] + w011 * c011[c] +
-             w101 * c101[c] + w111 * c111[c];
-    a0 = a[0];
-    a1 = a[1];
-    a2 = a[2];
-  }
-  adv[t][0] = a0;
-  adv[t][1] = a1;
-  adv[t][2] = a2;
-  __syncthreads();
-  const int n = min(CUB_TPB, K - (int)blockIdx.x * CUB_TPB);
-  const double* A = after + (size_t)blockIdx.x * CUB_TPB * r * 3;
-  for (int j = t; j < r; j += CUB_TPB) {
-    double s = 0.0;
-    for (int k = 0; k < n; k++) {
-      const double* m = A + ((size_t)k * r + j) * 3;
-      s += m[0] * adv[k][0] + m[1] * adv[k][1] + m[2] * adv[k][2];
+      for (int c = 0; c < 3; c++)
+        a[c] = w000 * c000[c] + w010 * c010[c] + w100 * c100[c] + w110 * c110[c] + w001 * c001[c] +
+               w011 * c011[c] + w101 * c101[c] + w111 * c111[c];
     }
-    partials[(size_t)blockIdx.x * r + j] = s;
+    adv[t][0] = a[0];
+    adv[t][1] = a[1];
+    adv[t][2] = a[2];
   }
+  __syncthreads();
+  const int j = blockIdx.y * CUB_TPB + t;
+  if (j >= r) return;
+  const int n = min(CUB_CELLS, K - i0);
+  const double* A = after + ((size_t)i0 * r + j) * 3;
+  double s = 0.0;
+  if (n == CUB_CELLS) {
+    double m[CUB_CELLS][3];
+#pragma unroll
+    for (int k = 0; k < CUB_CELLS; k++)
+#pragma unroll
+      for (int c = 0; c < 3; c++) m[k][c] = A[(size_t)k * r * 3 + c];
+#pragma unroll
+    for (int k = 0; k < CUB_CELLS; k++) s += m[k][0] * adv[k][0] + m[k][1] * adv[k][1] + m[k][2] * adv[k][2];
+  } else {
+    for (int k = 0; k < n; k++) {
+      const double* mk = A + (size_t)k * r * 3;
+      s += mk[0] * adv[k][0] + mk[1] * adv[k][1] + mk[2] * adv[k][2];
+    }
+  }
+  partials[(size_t)blockIdx.x * r + j] = s;
 }
 
 // q_out[j] = alpha * sum_w partials[w][j] + forcing[j] (forcing may be null)
@@ -111,9 +127,17 @@ __global__ void __launch_bounds__(256) cubature_reduce_kernel(const double* __re
                                                               double* __restrict__ q_out) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= r) return;
-  double s = 0.0;
-  for (int w = 0; w < nwg; w++) s += partials[(size_t)w * r + j];
-  q_out[j] = alpha * s + (forcing ? forcing[j] : 0.0);
+  // four running sums so that four loads are in flight, combined in a fixed order
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  int w = 0;
+  for (; w + 4 <= nwg; w += 4) {
+    s0 += partials[(size_t)w * r + j];
+    s1 += partials[(size_t)(w + 1) * r + j];
+    s2 += partials[(size_t)(w + 2) * r + j];
+    s3 += partials[(size_t)(w + 3) * r + j];
+  }
+  for (; w < nwg; w++) s0 += partials[(size_t)w * r + j];
+  q_out[j] = alpha * ((s0 + s1) + (s2 + s3)) + (forcing ? forcing[j] : 0.0);
 }
 
 // after[i][j][c] *= weights[i]
@@ -188,7 +212,7 @@ int cfs_cubature_create(int K, const int32_t* cells, const double* weights, cfs_
   C->X = info.dims[0];
   C->Y = info.dims[1];
   C->Z = info.dims[2];
-  C->nwg = (K + CUB_TPB - 1) / CUB_TPB;
+  C->nwg = (K + CUB_CELLS - 1) / CUB_CELLS;
   C->cells.assign(cells, cells + K);
   C->weights.assign(weights, weights + K);
   const size_t per = (size_t)3 * C->r;
@@ -281,8 +305,8 @@ int cfs_cubature_advect(cfs_cubature* c, cfs_basis* pre, const double* q_in, dou
   DeviceSwitch g(c->device);
   const hipStream_t s = (hipStream_t)stream;
   if ((st = cfs_unproject(pre, q_in, field, stream))) return st;
-  hipLaunchKernelGGL(cubature_kernel, dim3(c->nwg), dim3(CUB_TPB), 0, s, field, c->X, c->Y, c->Z, c->d_cells, c->K,
-                     c->d_after, c->r, (double)dt0, c->d_partials);
+  hipLaunchKernelGGL(cubature_kernel, dim3(c->nwg, (c->r + CUB_TPB - 1) / CUB_TPB), dim3(CUB_TPB), 0, s, field, c->X,
+                     c->Y, c->Z, c->d_cells, c->K, c->d_after, c->r, (double)dt0, c->d_partials);
   CUB_HIP(hipGetLastError(), "cubature launch");
   hipLaunchKernelGGL(cubature_reduce_kernel, dim3((c->r + 255) / 256), dim3(256), 0, s, c->d_partials, c->nwg, c->r,
                      alpha, forcing, q_out);

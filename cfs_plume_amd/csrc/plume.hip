@@ -1,0 +1,413 @@
+// plume.hip -- the whole compressed plume timestep on the device.
+//
+// stepPlume (reference src/integrators/SUBSPACE_FLUID_3D_COMPRESSED_EIGEN.cpp:
+// 453-501), preceded each frame by addSmokeColumn (FLUID_3D_MIC.cpp:655-703,
+// called from reducedCompressed3D.cpp:385), as launches on one stream with
+// no host round trip and no allocation, so a caller can capture it in a HIP
+// graph:
+//
+//   forces      velocity.setZeroBorder(); velocity += dt * buoyancy * heat (y)
+//               (addBuoyancy, FLUID_3D_MIC.cpp:491-502); vorticity confinement
+//               (addVorticity :524-593): the curl, its magnitude, then
+//               velocity += dt * eps * dx * (N x w) with N the normalised
+//               gradient of |w| -- two stencil kernels
+//   project     q = U_pre^T velocity                       (cfs_project)
+//   scalars     velocity border per axis (setZeroX/copyBorderY/setZeroZ for the
+//               plume's walls, VECTOR3_FIELD_3D.cpp:2872-3057); density and heat
+//               advected by velocity with dt0 = dt / dx, then their borders
+//               zeroed (advectHeatAndDensityStam :796-824)
+//   advection   q = Adv(q): the cubature (reducedAdvectCompressionFriendly),
+//               the learned closure, or the exact U^T A(U q)
+//   reduced     q = D q; q = P q + inverseProduct q  (reducedPeeledDiffusion
+//               :848-854, reducedStagedProject :942-946), one workgroup, FP64
+//   unproject   velocity interior = U_final q             (cfs_unproject)
+//
+// Fields are full grids X x Y x Z (the peeled basis dims + 2), x fastest,
+// FP32; velocity is interleaved xyz.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/cfs_plume.h"
+#include "basis_internal.h"
+
+namespace cfs {
+hipError_t launch_advect_scalar(const float* vel, const float* old, float* out, int X, int Y, int Z, float dt0,
+                                hipStream_t s);
+}  // namespace cfs
+
+namespace {
+
+constexpr int TPB = 256;
+constexpr int MAX_R = 256;
+
+__device__ __forceinline__ bool on_border(int x, int y, int z, int X, int Y, int Z) {
+  return x == 0 || y == 0 || z == 0 || x == X - 1 || y == Y - 1 || z == Z - 1;
+}
+
+// setZeroBorder of the velocity, then velocity += dt * (buoyancy * heat) e_y on
+// every cell (addBuoyancy adds to the force of every cell; the step adds
+// dt * force).
+__global__ void __launch_bounds__(TPB) buoyancy_kernel(float* __restrict__ vel, const float* __restrict__ heat, int X,
+                                                       int Y, int Z, float dt, float beta) {
+  const int i = blockIdx.x * TPB + threadIdx.x;
+  if (i >= X * Y * Z) return;
+  const int x = i % X, y = (i / X) % Y, z = i / (X * Y);
+  float* v = vel + 3 * (size_t)i;
+  if (on_border(x, y, z, X, Y, Z)) {
+    v[0] = 0.0f;
+    v[1] = 0.0f;
+    v[2] = 0.0f;
+  }
+  v[1] += dt * (beta * heat[i]);
+}
+
+// Neighbours of an interior cell for the vorticity stencils: the cell itself
+// in place of a neighbour on the border layer, and the matching inverse
+// spacings (one-sided 1/dx, central 0.5/dx).
+struct Stencil {
+  int up, down, out, in, right, left;
+  float dy, dz, dx;
+};
+__device__ __forceinline__ Stencil stencil(int i, int x, int y, int z, int X, int Y, int Z, float inv_dx) {
+  Stencil s;
+  const int slab = X * Y;
+  const float grid = 0.5f * inv_dx;
+  s.up = y == Y - 2 ? i : i + X;
+  s.down = y == 1 ? i : i - X;
+  s.dy = (s.up == i || s.down == i) ? inv_dx : grid;
+  s.out = z == Z - 2 ? i : i + slab;
+  s.in = z == 1 ? i : i - slab;
+  s.dz = (s.out == i || s.in == i) ? inv_dx : grid;
+  s.right = x == X - 2 ? i : i + 1;
+  s.left = x == 1 ? i : i - 1;
+  s.dx = (s.left == i || s.right == i) ? inv_dx : grid;
+  return s;
+}
+
+// the curl w of the velocity and |w| at interior cells (border entries stay 0)
+__global__ void __launch_bounds__(TPB) curl_kernel(const float* __restrict__ vel, float* __restrict__ w,
+                                                   float* __restrict__ wmag, int X, int Y, int Z, float inv_dx) {
+  const int i = blockIdx.x * TPB + threadIdx.x;
+  if (i >= X * Y * Z) return;
+  const int x = i % X, y = (i / X) % Y, z = i / (X * Y);
+  if (on_border(x, y, z, X, Y, Z)) return;
+  const Stencil s = stencil(i, x, y, z, X, Y, Z, inv_dx);
+  const float* v = vel;
+  const float w0 = (v[3 * s.up + 2] - v[3 * s.down + 2]) * s.dy + (-v[3 * s.out + 1] + v[3 * s.in + 1]) * s.dz;
+  const float w1 = (v[3 * s.out + 0] - v[3 * s.in + 0]) * s.dz + (-v[3 * s.right + 2] + v[3 * s.left + 2]) * s.dx;
+  const float w2 = (v[3 * s.right + 1] - v[3 * s.left + 1]) * s.dx + (-v[3 * s.up + 0] + v[3 * s.down + 0]) * s.dy;
+  w[3 * (size_t)i] = w0;
+  w[3 * (size_t)i + 1] = w1;
+  w[3 * (size_t)i + 2] = w2;
+  wmag[i] = sqrtf(w0 * w0 + w1 * w1 + w2 * w2);
+}
+
+// velocity += dt * eps * dx * (N x w) at interior cells, N = grad|w| / |grad|w||
+__global__ void __launch_bounds__(TPB) confinement_kernel(float* __restrict__ vel, const float* __restrict__ w,
+                                                          const float* __restrict__ wmag, int X, int Y, int Z,
+                                                          float inv_dx, float dt, float eps_dx) {
+  const int i = blockIdx.x * TPB + threadIdx.x;
+  if (i >= X * Y * Z) return;
+  const int x = i % X, y = (i / X) % Y, z = i / (X * Y);
+  if (on_border(x, y, z, X, Y, Z)) return;
+  const Stencil s = stencil(i, x, y, z, X, Y, Z, inv_dx);
+  float n0 = (wmag[s.right] - wmag[s.left]) * s.dx;
+  float n1 = (wmag[s.up] - wmag[s.down]) * s.dy;
+  float n2 = (wmag[s.out] - wmag[s.in]) * s.dz;
+  float m = sqrtf(n0 * n0 + n1 * n1 + n2 * n2);
+  if (m > 0.0f) {
+    m = 1.0f / m;
+    n0 *= m;
+    n1 *= m;
+    n2 *= m;
+    const float w0 = w[3 * (size_t)i], w1 = w[3 * (size_t)i + 1], w2 = w[3 * (size_t)i + 2];
+    float* v = vel + 3 * (size_t)i;
+    v[0] += dt * ((n1 * w2 - n2 * w1) * eps_dx);
+    v[1] += dt * (-(n0 * w2 - n2 * w0) * eps_dx);
+    v[2] += dt * ((n0 * w1 - n1 * w0) * eps_dx);
+  }
+}
+
+// The velocity border before the scalar advection: per axis, a wall zeroes
+// the normal component on its two faces (setZeroX/Y/Z), an open side copies
+// it from the neighbouring layer (copyBorderX/Y/Z).
+__global__ void __launch_bounds__(TPB) velocity_border_kernel(float* __restrict__ vel, int X, int Y, int Z,
+                                                              int open_x, int open_y, int open_z) {
+  const int i = blockIdx.x * TPB + threadIdx.x;
+  if (i >= X * Y * Z) return;
+  const int x = i % X, y = (i / X) % Y, z = i / (X * Y);
+  float* v = vel + 3 * (size_t)i;
+  if (x == 0 || x == X - 1) v[0] = open_x ? vel[3 * (size_t)(x == 0 ? i + 1 : i - 1)] : 0.0f;
+  if (y == 0 || y == Y - 1) v[1] = open_y ? vel[3 * (size_t)(y == 0 ? i + X : i - X) + 1] : 0.0f;
+  if (z == 0 || z == Z - 1) {
+    const int slab = X * Y;
+    v[2] = open_z ? vel[3 * (size_t)(z == 0 ? i + slab : i - slab) + 2] : 0.0f;
+  }
+}
+
+__global__ void __launch_bounds__(TPB) copy2_kernel(const float* __restrict__ a, float* __restrict__ a_old,
+                                                    const float* __restrict__ b, float* __restrict__ b_old, int n) {
+  const int i = blockIdx.x * TPB + threadIdx.x;
+  if (i >= n) return;
+  a_old[i] = a[i];
+  b_old[i] = b[i];
+}
+
+// FIELD_3D::setZeroBorder of density and heat
+__global__ void __launch_bounds__(TPB) zero_border2_kernel(float* __restrict__ a, float* __restrict__ b, int X, int Y,
+                                                           int Z) {
+  const int i = blockIdx.x * TPB + threadIdx.x;
+  if (i >= X * Y * Z) return;
+  const int x = i % X, y = (i / X) % Y, z = i / (X * Y);
+  if (on_border(x, y, z, X, Y, Z)) {
+    a[i] = 0.0f;
+    b[i] = 0.0f;
+  }
+}
+
+// addSmokeTestCase (FLUID_3D_MIC.cpp:674-703) on density and heat: 1 inside
+// the column of radius 0.075 xTotal around (0.4 xTotal, 0.5 zTotal) for
+// y in [(int)(0.05 Y), (int)(0.10 Y)], lengths in units of dx = 1 / maxRes.
+__global__ void __launch_bounds__(TPB) smoke_kernel(float* __restrict__ dens, float* __restrict__ heat, int X, int Y,
+                                                    int Z, int y0, int y1) {
+  const int i = blockIdx.x * TPB + threadIdx.x;
+  if (i >= X * Y * Z) return;
+  const int x = i % X, y = (i / X) % Y, z = i / (X * Y);
+  if (y < y0 || y > y1) return;
+  const int maxres = max(X, max(Y, Z));
+  const double dx = 1.0 / (double)(float)maxres;
+  const double xtot = dx * X, ztot = dx * Z;
+  const double xl = x * dx - xtot * (double)0.4f;
+  const double zl = z * dx - ztot * (double)0.5f;
+  const float radius = sqrtf((float)(xl * xl + zl * zl));
+  if ((double)radius < (double)0.075f * xtot) {
+    dens[i] = 1.0f;
+    heat[i] = 1.0f;
+  }
+}
+
+// q <- D q, then q <- P q + I q (FP64), r <= 256, one workgroup.  q_in and
+// q_out may alias.
+__global__ void __launch_bounds__(MAX_R) reduced_ops_kernel(const double* __restrict__ D, const double* __restrict__ P,
+                                                            const double* __restrict__ I, const double* q_in,
+                                                            double* q_out, int r) {
+  __shared__ double a[MAX_R], t[MAX_R];
+  const int j = threadIdx.x;
+  if (j < r) a[j] = q_in[j];
+  __syncthreads();
+  if (j < r) {
+    double s = 0.0;
+    for (int k = 0; k < r; k++) s += D[(size_t)j * r + k] * a[k];
+    t[j] = s;
+  }
+  __syncthreads();
+  if (j < r) {
+    double p = 0.0, c = 0.0;
+    for (int k = 0; k < r; k++) {
+      p += P[(size_t)j * r + k] * t[k];
+      c += I[(size_t)j * r + k] * t[k];
+    }
+    q_out[j] = p + c;
+  }
+}
+
+}  // namespace
+
+struct cfs_plume {
+  int device = 0;
+  cfs_basis* pre = nullptr;
+  cfs_basis* fin = nullptr;
+  cfs_plume_params prm{};
+  int X = 0, Y = 0, Z = 0, r = 0;  // full grid, columns
+  int mode = CFS_PLUME_ADVECT_EXACT;
+  cfs_cubature* cub = nullptr;
+  cfs_closure* closure = nullptr;
+  double *d_D = nullptr, *d_P = nullptr, *d_I = nullptr, *d_q = nullptr;
+  float *d_w = nullptr, *d_wmag = nullptr, *d_dold = nullptr, *d_hold = nullptr, *d_fa = nullptr, *d_fb = nullptr;
+  std::vector<void*> allocs;
+  ~cfs_plume() {
+    for (void* p : allocs) (void)hipFree(p);
+  }
+};
+
+namespace {
+
+struct DeviceSwitch {
+  int prev = -1;
+  explicit DeviceSwitch(int dev) {
+    if (hipGetDevice(&prev) == hipSuccess && prev != dev) (void)hipSetDevice(dev);
+    else prev = -1;
+  }
+  ~DeviceSwitch() {
+    if (prev >= 0) (void)hipSetDevice(prev);
+  }
+};
+
+template <class T>
+int dev_alloc(cfs_plume* p, T** out, size_t count, const T* host) {
+  const hipError_t e = hipMalloc(reinterpret_cast<void**>(out), std::max<size_t>(count, 1) * sizeof(T));
+  if (e != hipSuccess) return cfs::hip_fail(e, "hipMalloc");
+  p->allocs.push_back(*out);
+  const hipError_t e2 = host ? hipMemcpy(*out, host, count * sizeof(T), hipMemcpyHostToDevice)
+                             : hipMemset(*out, 0, count * sizeof(T));
+  if (e2 != hipSuccess) return cfs::hip_fail(e2, host ? "hipMemcpy" : "hipMemset");
+  return CFS_OK;
+}
+
+#define PL_HIP(call, what)                                  \
+  do {                                                      \
+    const hipError_t e_ = (call);                           \
+    if (e_ != hipSuccess) return cfs::hip_fail(e_, what);   \
+  } while (0)
+
+}  // namespace
+
+extern "C" {
+
+void cfs_plume_default_params(int X, int Y, int Z, cfs_plume_params* prm) {
+  if (!prm) return;
+  // FLUID_3D_MIC constructor (FLUID_3D_MIC.cpp:30-56): dt 0.1, buoyancy 0.1,
+  // vorticity 2 / max(1, 64 / maxRes), dx = 1 / maxRes; the plume driver's
+  // walls front/back/left/right, open top/bottom (reducedCompressed3D.cpp:325-331)
+  const int maxres = std::max(X, std::max(Y, Z));
+  prm->dt = 0.1;
+  prm->dx = 1.0 / (double)(float)maxres;
+  prm->buoyancy = 0.1;
+  const double scaling = std::max(1.0, (double)(64.0f / (float)maxres));
+  prm->vorticity_eps = 2.0 / scaling;
+  prm->open_x = 0;
+  prm->open_y = 1;
+  prm->open_z = 0;
+}
+
+int cfs_plume_create(cfs_basis* pre, cfs_basis* fin, const double* damping, const double* ptof,
+                     const double* inverse_product, const cfs_plume_params* prm, cfs_plume** out) {
+  if (!out) return cfs::fail(CFS_ERR_ARG, "null output handle");
+  *out = nullptr;
+  if (!pre || !fin || !damping || !ptof || !inverse_product || !prm) return cfs::fail(CFS_ERR_ARG, "null argument");
+  cfs_basis_info a, b;
+  int st = cfs_basis_get_info(pre, &a);
+  if (st || (st = cfs_basis_get_info(fin, &b))) return st;
+  if (a.dims[0] != b.dims[0] || a.dims[1] != b.dims[1] || a.dims[2] != b.dims[2] || a.num_cols != b.num_cols)
+    return cfs::fail(CFS_ERR_ARG, "U_preadvect and U_final must share the grid and the column count");
+  if (a.num_cols > MAX_R) return cfs::fail(CFS_ERR_ARG, "more than 256 columns");
+  if (cfs::basis_device(pre) != cfs::basis_device(fin)) return cfs::fail(CFS_ERR_ARG, "bases on different devices");
+  if (!(prm->dt > 0.0) || !(prm->dx > 0.0)) return cfs::fail(CFS_ERR_ARG, "dt and dx must be positive");
+  std::unique_ptr<cfs_plume> P(new cfs_plume);
+  P->device = cfs::basis_device(pre);
+  DeviceSwitch g(P->device);
+  P->pre = pre;
+  P->fin = fin;
+  P->prm = *prm;
+  P->X = a.dims[0] + 2;
+  P->Y = a.dims[1] + 2;
+  P->Z = a.dims[2] + 2;
+  P->r = a.num_cols;
+  const size_t n = (size_t)P->X * P->Y * P->Z, rr = (size_t)P->r * P->r;
+  if ((st = dev_alloc(P.get(), &P->d_D, rr, damping))) return st;
+  if ((st = dev_alloc(P.get(), &P->d_P, rr, ptof))) return st;
+  if ((st = dev_alloc(P.get(), &P->d_I, rr, inverse_product))) return st;
+  if ((st = dev_alloc<double>(P.get(), &P->d_q, (size_t)P->r, nullptr))) return st;
+  if ((st = dev_alloc<float>(P.get(), &P->d_w, 3 * n, nullptr))) return st;
+  if ((st = dev_alloc<float>(P.get(), &P->d_wmag, n, nullptr))) return st;
+  if ((st = dev_alloc<float>(P.get(), &P->d_dold, n, nullptr))) return st;
+  if ((st = dev_alloc<float>(P.get(), &P->d_hold, n, nullptr))) return st;
+  if ((st = dev_alloc<float>(P.get(), &P->d_fa, 3 * n, nullptr))) return st;
+  if ((st = dev_alloc<float>(P.get(), &P->d_fb, 3 * n, nullptr))) return st;
+  *out = P.release();
+  return CFS_OK;
+}
+
+int cfs_plume_set_advection(cfs_plume* p, int mode, cfs_cubature* cub, cfs_closure* closure) {
+  if (!p) return cfs::fail(CFS_ERR_ARG, "null plume");
+  if (mode == CFS_PLUME_ADVECT_CUBATURE) {
+    int K = 0, r = 0;
+    if (!cub) return cfs::fail(CFS_ERR_ARG, "cubature mode needs a cubature");
+    int st = cfs_cubature_get_info(cub, &K, &r);
+    if (st) return st;
+    if (r != p->r) return cfs::fail(CFS_ERR_ARG, "the cubature's after-basis has another column count");
+  } else if (mode == CFS_PLUME_ADVECT_CLOSURE) {
+    int r = 0, rp = 0, h = 0;
+    int64_t np = 0;
+    if (!closure) return cfs::fail(CFS_ERR_ARG, "closure mode needs a closure");
+    int st = cfs_closure_get_dims(closure, &r, &rp, &h, &np);
+    if (st) return st;
+    if (r != p->r) return cfs::fail(CFS_ERR_ARG, "the closure has another column count");
+  } else if (mode != CFS_PLUME_ADVECT_EXACT) {
+    return cfs::fail(CFS_ERR_ARG, "unknown advection mode");
+  }
+  p->mode = mode;
+  p->cub = cub;
+  p->closure = closure;
+  return CFS_OK;
+}
+
+int cfs_plume_add_smoke(cfs_plume* p, float* density, float* heat, void* stream) {
+  if (!p || !density || !heat) return cfs::fail(CFS_ERR_ARG, "null argument");
+  DeviceSwitch g(p->device);
+  const int n = p->X * p->Y * p->Z;
+  const int y0 = (int)(0.05 * p->Y), y1 = (int)(0.10 * p->Y);
+  hipLaunchKernelGGL(smoke_kernel, dim3((n + TPB - 1) / TPB), dim3(TPB), 0, (hipStream_t)stream, density, heat, p->X,
+                     p->Y, p->Z, y0, y1);
+  PL_HIP(hipGetLastError(), "smoke launch");
+  return CFS_OK;
+}
+
+int cfs_plume_step(cfs_plume* p, float* velocity, float* density, float* heat, double* q, void* stream) {
+  if (!p || !velocity || !density || !heat || !q) return cfs::fail(CFS_ERR_ARG, "null argument");
+  DeviceSwitch g(p->device);
+  const hipStream_t s = (hipStream_t)stream;
+  const int X = p->X, Y = p->Y, Z = p->Z, n = X * Y * Z;
+  const dim3 grid((n + TPB - 1) / TPB), block(TPB);
+  const float dt = (float)p->prm.dt, inv_dx = (float)(1.0 / p->prm.dx);
+  const float dt0 = (float)(p->prm.dt / p->prm.dx);
+  int st;
+  // forces
+  hipLaunchKernelGGL(buoyancy_kernel, grid, block, 0, s, velocity, heat, X, Y, Z, dt, (float)p->prm.buoyancy);
+  PL_HIP(hipGetLastError(), "buoyancy launch");
+  if (p->prm.vorticity_eps > 0.0) {
+    hipLaunchKernelGGL(curl_kernel, grid, block, 0, s, velocity, p->d_w, p->d_wmag, X, Y, Z, inv_dx);
+    PL_HIP(hipGetLastError(), "curl launch");
+    hipLaunchKernelGGL(confinement_kernel, grid, block, 0, s, velocity, p->d_w, p->d_wmag, X, Y, Z, inv_dx, dt,
+                       (float)(p->prm.vorticity_eps * p->prm.dx));
+    PL_HIP(hipGetLastError(), "confinement launch");
+  }
+  // projection into the subspace
+  if ((st = cfs_project(p->pre, velocity, q, stream))) return st;
+  // density and heat
+  hipLaunchKernelGGL(velocity_border_kernel, grid, block, 0, s, velocity, X, Y, Z, p->prm.open_x, p->prm.open_y,
+                     p->prm.open_z);
+  PL_HIP(hipGetLastError(), "border launch");
+  hipLaunchKernelGGL(copy2_kernel, grid, block, 0, s, density, p->d_dold, heat, p->d_hold, n);
+  PL_HIP(hipGetLastError(), "copy launch");
+  PL_HIP(cfs::launch_advect_scalar(velocity, p->d_dold, density, X, Y, Z, dt0, s), "advect launch");
+  PL_HIP(cfs::launch_advect_scalar(velocity, p->d_hold, heat, X, Y, Z, dt0, s), "advect launch");
+  hipLaunchKernelGGL(zero_border2_kernel, grid, block, 0, s, density, heat, X, Y, Z);
+  PL_HIP(hipGetLastError(), "border launch");
+  // advection in the subspace
+  if (p->mode == CFS_PLUME_ADVECT_CUBATURE)
+    st = cfs_cubature_advect(p->cub, p->pre, q, p->d_q, p->d_fa, dt0, 1.0, nullptr, stream);
+  else if (p->mode == CFS_PLUME_ADVECT_CLOSURE)
+    st = cfs_closure_step(p->closure, q, p->d_q, 1, 1.0, nullptr, stream);
+  else
+    st = cfs_reduced_advect(p->pre, q, p->d_q, p->d_fa, p->d_fb, dt0, 1.0, nullptr, stream);
+  if (st) return st;
+  // diffusion and pressure projection
+  hipLaunchKernelGGL(reduced_ops_kernel, dim3(1), dim3(MAX_R), 0, s, p->d_D, p->d_P, p->d_I, p->d_q, q, p->r);
+  PL_HIP(hipGetLastError(), "reduced operators launch");
+  // unprojection
+  return cfs_unproject(p->fin, q, velocity, stream);
+}
+
+int cfs_plume_destroy(cfs_plume* p) {
+  if (!p) return cfs::fail(CFS_ERR_ARG, "null plume");
+  DeviceSwitch g(p->device);
+  delete p;
+  return CFS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,100 @@
+"""The whole compressed plume timestep on the device (host side over the C ABI).
+
+Reference (aadjones/CFS_PLUME) -> here:
+
+  FLUID_3D_MIC constants (FLUID_3D_MIC.cpp:30-56)            default_params
+  addSmokeColumn (FLUID_3D_MIC.cpp:655-703)                 PlumeSolver.add_smoke (cfs_plume_add_smoke)
+  stepPlume (SUBSPACE_FLUID_3D_COMPRESSED_EIGEN.cpp:453-501) PlumeSolver.step (cfs_plume_step)
+
+A step is kernel launches on one stream (forces, project, density and heat
+advection, subspace advection, reduced diffusion and pressure projection,
+unproject), so it can be captured in a HIP graph (PlumeSolver.capture).
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Optional
+
+import numpy as np
+
+from . import _lib
+from .basis import CompressedBasis, _sptr
+from .hip import DeviceArray, Graph, Stream
+
+ADVECT_EXACT, ADVECT_CUBATURE, ADVECT_CLOSURE = 0, 1, 2
+
+
+def default_params(full_dims) -> _lib.PlumeParams:
+    """dt 0.1, dx 1 / max resolution, buoyancy 0.1, vorticity 2 / max(1, 64 /
+    max resolution); walls on x and z, open y (the plume driver's sides)."""
+    p = _lib.PlumeParams()
+    X, Y, Z = full_dims
+    _lib.lib().cfs_plume_default_params(X, Y, Z, ctypes.byref(p))
+    return p
+
+
+class PlumeSolver:
+    """stepPlume with U_preadvect, U_final and the reduced matrices on the device."""
+
+    def __init__(self, pre: CompressedBasis, fin: CompressedBasis, damping: np.ndarray, ptof: np.ndarray,
+                 inverse_product: np.ndarray, params: Optional[_lib.PlumeParams] = None):
+        X, Y, Z = pre.dims
+        self.full = (X + 2, Y + 2, Z + 2)
+        self.r = pre.r
+        self.params = params if params is not None else default_params(self.full)
+        mats = [np.ascontiguousarray(m, np.float64) for m in (damping, ptof, inverse_product)]
+        for m in mats:
+            if m.shape != (self.r, self.r):
+                raise ValueError(f"reduced matrices must be {self.r} x {self.r}, got {m.shape}")
+        h = ctypes.c_void_p()
+        _lib.check(_lib.lib().cfs_plume_create(pre._h, fin._h, mats[0].ctypes.data, mats[1].ctypes.data,
+                                               mats[2].ctypes.data, ctypes.byref(self.params), ctypes.byref(h)),
+                   "cfs_plume_create")
+        self._h = h.value
+        self.pre, self.fin = pre, fin
+        self._keep = ()
+
+    @classmethod
+    def from_operators(cls, pre: CompressedBasis, fin: CompressedBasis, ops, params=None) -> "PlumeSolver":
+        """With a ReducedOperators (the reference's .matrix files)."""
+        return cls(pre, fin, ops.damping, ops.ptof, ops.inverse_product, params)
+
+    def set_advection(self, mode: int, cubature=None, closure=None):
+        _lib.check(_lib.lib().cfs_plume_set_advection(self._h, mode, cubature._h if cubature else None,
+                                                      closure.handle if closure else None), "cfs_plume_set_advection")
+        self._keep = (cubature, closure)
+
+    def fields(self):
+        """Zeroed device state: velocity (3 per cell), density, heat, q."""
+        n = self.full[0] * self.full[1] * self.full[2]
+        return (DeviceArray.zeros((3 * n,), np.float32), DeviceArray.zeros((n,), np.float32),
+                DeviceArray.zeros((n,), np.float32), DeviceArray.zeros((self.r,), np.float64))
+
+    def add_smoke(self, density: DeviceArray, heat: DeviceArray, stream: Optional[Stream] = None):
+        _lib.check(_lib.lib().cfs_plume_add_smoke(self._h, density.ptr, heat.ptr, _sptr(stream)), "addSmokeColumn")
+
+    def step(self, velocity: DeviceArray, density: DeviceArray, heat: DeviceArray, q: DeviceArray,
+             stream: Optional[Stream] = None):
+        _lib.check(_lib.lib().cfs_plume_step(self._h, velocity.ptr, density.ptr, heat.ptr, q.ptr, _sptr(stream)),
+                   "stepPlume")
+
+    def frame(self, velocity, density, heat, q, stream: Optional[Stream] = None):
+        """One frame of the driver: addSmokeColumn, then stepPlume
+        (reducedCompressed3D.cpp:385-386)."""
+        self.add_smoke(density, heat, stream)
+        self.step(velocity, density, heat, q, stream)
+
+    def capture(self, velocity, density, heat, q, stream: Stream) -> Graph:
+        """A HIP graph of one frame on these arrays."""
+        return Graph(stream, lambda: self.frame(velocity, density, heat, q, stream))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.lib().cfs_plume_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -258,6 +258,43 @@ int cfs_closure_train_step(cfs_closure* c, const float* x, const float* t, const
 int cfs_closure_step(cfs_closure* c, const double* q_in, double* q_out, int n, double alpha, const double* forcing,
                      void* stream);
 
+/* ---- the whole plume timestep on the device --------------------------------
+ * stepPlume (SUBSPACE_FLUID_3D_COMPRESSED_EIGEN.cpp:453-501) without a host
+ * round trip: buoyancy and vorticity confinement on the full grid
+ * (FLUID_3D_MIC.cpp:491-593), q = U_pre^T v, the velocity border and the
+ * density/heat advection (advectHeatAndDensityStam :796-824), the subspace
+ * advection (cubature, learned closure or exact), q = D q then
+ * q = P q + inverseProduct q (:848-854, :942-946), and the unprojection
+ * v = U_final q.  Kernel launches only: a caller may capture a step (and
+ * cfs_plume_add_smoke, addSmokeColumn FLUID_3D_MIC.cpp:655-703) in a HIP
+ * graph.  Fields are the caller's full grids (velocity interleaved xyz,
+ * density and heat scalar, FP32); q is FP64[r]. */
+typedef struct cfs_plume cfs_plume;
+typedef struct {
+  double dt;             /* time step (_dt) */
+  double dx;             /* cell size (_dx = 1 / max resolution) */
+  double buoyancy;       /* _buoyancy */
+  double vorticity_eps;  /* _vorticityEps after its resolution scaling; 0 turns confinement off */
+  int open_x, open_y, open_z;  /* 1: open side, the border copies its neighbour
+                                  (copyBorderX/Y/Z); 0: wall, the normal component is zeroed */
+} cfs_plume_params;
+#define CFS_PLUME_ADVECT_EXACT 0
+#define CFS_PLUME_ADVECT_CUBATURE 1
+#define CFS_PLUME_ADVECT_CLOSURE 2
+/* The FLUID_3D_MIC constants for a full grid X x Y x Z and the plume
+ * driver's walls (front/back/left/right walls, open top and bottom,
+ * reducedCompressed3D.cpp:325-331). */
+void cfs_plume_default_params(int X, int Y, int Z, cfs_plume_params* params);
+/* damping, ptof, inverse_product: host FP64 r x r row-major (the
+ * damping.peeled, projected.ptof and inverseProduct matrices). */
+int cfs_plume_create(cfs_basis* pre, cfs_basis* fin, const double* damping, const double* ptof,
+                     const double* inverse_product, const cfs_plume_params* params, cfs_plume** out);
+/* Selects the subspace advection; the cubature or closure must outlive the plume. */
+int cfs_plume_set_advection(cfs_plume* p, int mode, cfs_cubature* cubature, cfs_closure* closure);
+int cfs_plume_add_smoke(cfs_plume* p, float* density, float* heat, void* stream);
+int cfs_plume_step(cfs_plume* p, float* velocity, float* density, float* heat, double* q, void* stream);
+int cfs_plume_destroy(cfs_plume* p);
+
 /* ---- encoder (host) ---------------------------------------------------------
  * CompressAndWriteMatrixComponents (COMPRESSION.cpp:2271-2359) with the SVD
  * transform off, as on the live path (:2312).  U is FP64 column-major:
