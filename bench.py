@@ -36,7 +36,7 @@ WORKLOADS = {
 }
 DEFAULT = "128x192x128_r200"
 BATCH = "batch16"  # 16 independent bases of seeded sizes in one persistent launch
-CLOSURE = "closure"  # learned advection closure vs the exact reduced-advection step, 48x64x48 r=52
+CLOSURE = "closure"  # learned advection closure vs the cubature reduced-advection step, 48x64x48 r=52
 ACCURACY = "accuracy"  # compressed vs uncompressed reduced plume, per-step q error, 48x64x48 r=52
 
 
@@ -262,7 +262,9 @@ def run_closure(args):
             "grid": list(grid), "r": r, "parallelism": "single GPU",
         },
         "cubature": {
-            "key_cells": int(len(res.cells)), "passes": res.passes, "training_rel_error": res.relative_error,
+            "key_cells": int(len(res.cells)), "passes": res.passes,
+            "training_rel_error": None if args.cubature_file else res.relative_error,
+            "file": os.path.basename(args.cubature_file) if args.cubature_file else None,
             "samples": int(len(states)), "candidates_per_pass": args.cubature_candidates,
             "tolerance": args.cubature_tolerance,
             "fit_s": round(t_cub, 2), "drift_vs_exact_at_100": round(d_cub, 6),
@@ -391,7 +393,7 @@ def main():
     ap.add_argument("--accuracy-out", default="",
                     help="write each accuracy run's trajectories as <prefix>.<i>[.compressed].qDot.matrix")
     ap.add_argument("--closure-runs", type=int, default=200, help="training runs of the closure workload")
-    ap.add_argument("--closure-epochs", type=int, default=300)
+    ap.add_argument("--closure-epochs", type=int, default=600)
     ap.add_argument("--cubature-samples", type=int, default=160,
                     help="exact reduced-advection states the closure workload's cubature is fitted to")
     ap.add_argument("--cubature-runs", type=int, default=120, help="exact runs the cubature's states come from")
@@ -399,7 +401,7 @@ def main():
     ap.add_argument("--cubature-tolerance", type=float, default=1e-3, help="relative fit error that ends the passes")
     ap.add_argument("--cubature-file", default="",
                     help="read the closure workload's cubature from this file instead of fitting one")
-    ap.add_argument("--closure-noise", type=float, default=0.1,
+    ap.add_argument("--closure-noise", type=float, default=0.2,
                     help="relative size of the perturbed states added to the closure's training data")
     ap.add_argument("--cpu-sample-cols", type=int, default=200,
                     help="columns of the oracle CPU step to time (the full step when >= r)")
