@@ -37,6 +37,7 @@ WORKLOADS = {
 DEFAULT = "128x192x128_r200"
 BATCH = "batch16"  # 16 independent bases of seeded sizes in one persistent launch
 CLOSURE = "closure"  # learned advection closure vs the cubature reduced-advection step, 48x64x48 r=52
+PLUME = "plume"  # the whole plume timestep on the device (stepPlume), two bases, HIP-graph replay
 ACCURACY = "accuracy"  # compressed vs uncompressed reduced plume, per-step q error, 48x64x48 r=52
 
 
@@ -286,6 +287,75 @@ def run_closure(args):
     print(json.dumps(result), flush=True)
 
 
+def run_plume(args):
+    """The timestep an integrator runs (addSmokeColumn + stepPlume): forces,
+    project with U_preadvect, density/heat advection, the subspace advection,
+    reduced diffusion and pressure projection, unproject with U_final -- two
+    passes over two different compressed bases, captured in one HIP graph.
+    Default grid 128x192x128, r = 200 (both bases synthetic, seeds 7 and 8)
+    with the exact reduced advection U^T A(U q) (two more passes); at
+    48x64x48, r = 52 also with the committed cubature fixture."""
+    import cfs_plume_amd as C
+    from cfs_plume_amd import cubature as CB
+    from cfs_plume_amd import plume as PL
+    from cfs_plume_amd import synth
+    from cfs_plume_amd.hip import DeviceArray, Stream
+
+    name = args.plume_grid
+    grid, r, percent = WORKLOADS[name]
+    pre = C.CompressedBasis.from_buffers(load_or_encode(C, synth, grid, r, percent, 7, name, args.cache,
+                                                        args.threads))
+    fin = C.CompressedBasis.from_buffers(load_or_encode(C, synth, grid, r, percent, 8, name + "_fin", args.cache,
+                                                        args.threads))
+    rng = np.random.default_rng(5)
+    D = 0.97 * np.eye(r) + 0.01 * rng.standard_normal((r, r)) / np.sqrt(r)
+    P = np.eye(r) + 0.01 * rng.standard_normal((r, r)) / np.sqrt(r)
+    I = 0.02 * rng.standard_normal((r, r)) / np.sqrt(r)
+    solver = PL.PlumeSolver(pre, fin, D, P, I)
+    modes = [("exact", PL.ADVECT_EXACT, None)]
+    fixture = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden", "cubature_48x64x48_r52")
+    if name == "48x64x48_r52" and os.path.exists(fixture):
+        modes.append(("cubature", PL.ADVECT_CUBATURE, CB.Cubature.from_file(fixture, pre)))
+    stream = Stream()
+    out = {}
+    for label, mode, cub in modes:
+        solver.set_advection(mode, cubature=cub)
+        vel, dens, heat, q = solver.fields()
+        for _ in range(3):
+            solver.frame(vel, dens, heat, q, stream)
+        graph = solver.capture(vel, dens, heat, q, stream)
+        ms, per = time_launches(graph.launch, args.steps, args.warmup, stream)
+        pre.raise_on_stream_errors()
+        qn = q.numpy()
+        out[label] = {"ms_per_frame": round(ms, 5), "median_ms": round(float(np.median(per)), 5),
+                      "frames_per_s": round(1000.0 / ms, 1), "q_norm": float(np.linalg.norm(qn))}
+        log(f"[bench] plume frame ({label} advection), {name}: {ms:.4f} ms (median {np.median(per):.4f}), "
+            f"|q| {np.linalg.norm(qn):.3e}")
+    # the step's two compressed passes alone
+    dv = DeviceArray.zeros((3 * solver.full[0] * solver.full[1] * solver.full[2],), np.float32)
+    qa = DeviceArray.from_numpy(rng.standard_normal(r))
+    qb = DeviceArray.zeros((r,), np.float64)
+    ms_p, _ = time_launches(lambda: pre.project(dv, qb, stream), args.steps, args.warmup, stream)
+    ms_u, _ = time_launches(lambda: fin.unproject(qa, dv, stream), args.steps, args.warmup, stream)
+    first = modes[0][0]
+    result = {
+        "metric": f"plume timesteps/s (addSmokeColumn + stepPlume on the device, {first} advection), "
+                  f"r={r} @{grid[0]}x{grid[1]}x{grid[2]}",
+        "value": out[first]["frames_per_s"], "unit": "frames/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": out[first]["ms_per_frame"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp32 fields, fp64 q", "data": "synthetic",
+        "config": {
+            "workload": f"stepPlume with U_preadvect != U_final (synthetic, seeds 7 and 8), grid "
+                        f"{grid[0]}x{grid[1]}x{grid[2]}, r={r}, percent={percent}; one HIP graph per frame",
+            "grid": list(grid), "r": r, "parallelism": "single GPU",
+        },
+        "frames": out,
+        "passes_ms": {"project_pre": round(ms_p, 5), "unproject_fin": round(ms_u, 5)},
+    }
+    solver.close()
+    print(json.dumps(result), flush=True)
+
+
 def run_accuracy(args):
     """Compression error of the reduced plume (the measurement of the
     reference's reducedCompressed3DCompareVelocities): the 48x64x48, r = 52
@@ -388,7 +458,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default=DEFAULT, choices=sorted(WORKLOADS) + [BATCH, CLOSURE, ACCURACY])
+    ap.add_argument("--workload", default=DEFAULT, choices=sorted(WORKLOADS) + [BATCH, CLOSURE, ACCURACY, PLUME])
+    ap.add_argument("--plume-grid", default=DEFAULT, choices=sorted(WORKLOADS),
+                    help="grid and r of the plume workload")
     ap.add_argument("--accuracy-runs", type=int, default=4, help="seeded runs of the accuracy workload")
     ap.add_argument("--accuracy-out", default="",
                     help="write each accuracy run's trajectories as <prefix>.<i>[.compressed].qDot.matrix")
@@ -427,6 +499,9 @@ def main():
         return
     if args.workload == ACCURACY:
         run_accuracy(args)
+        return
+    if args.workload == PLUME:
+        run_plume(args)
         return
 
     import cfs_plume_amd as C
