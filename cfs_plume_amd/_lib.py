@@ -75,6 +75,14 @@ SIGNATURES = {
     "cfs_plume_add_smoke": (_i, [_vp, _vp, _vp, _vp]),
     "cfs_plume_step": (_i, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "cfs_plume_destroy": (_i, [_vp]),
+    "cfs_fluid_default_params": (None, [_i, _i, _i, _vp]),
+    "cfs_fluid_create": (_i, [_i, _i, _i, _vp, ctypes.POINTER(_vp)]),
+    "cfs_fluid_fields": (_i, [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                              ctypes.POINTER(_vp)]),
+    "cfs_fluid_add_smoke": (_i, [_vp, _vp]),
+    "cfs_fluid_step": (_i, [_vp, _vp, _vp]),
+    "cfs_fluid_last_solve": (_i, [_vp, ctypes.POINTER(_i), ctypes.POINTER(ctypes.c_double)]),
+    "cfs_fluid_destroy": (_i, [_vp]),
     "cfs_encode_matrix": (_i, [_vp, _i, _i, _i, _i, _i, ctypes.c_double, _i, _i, ctypes.POINTER(_vp)]),
     "cfs_encode_block": (_i, [_vp, _i, ctypes.c_double, _i, ctypes.POINTER(ctypes.c_double),
                               ctypes.POINTER(ctypes.c_double), _vp, ctypes.POINTER(_i)]),
@@ -94,6 +102,24 @@ class PlumeParams(ctypes.Structure):
         ("open_y", ctypes.c_int),
         ("open_z", ctypes.c_int),
     ]
+
+
+class FluidParams(ctypes.Structure):
+    _fields_ = [
+        ("dt", ctypes.c_double),
+        ("dx", ctypes.c_double),
+        ("buoyancy", ctypes.c_double),
+        ("vorticity_eps", ctypes.c_double),
+        ("damping_w", ctypes.c_double),
+        ("solver_eps", ctypes.c_double),
+        ("iterations", ctypes.c_int),
+        ("boundaries", ctypes.c_int * 6),
+    ]
+
+
+class FluidSnapshots(ctypes.Structure):
+    _fields_ = [(name, ctypes.c_void_p) for name in
+                ("preadvect", "prediffuse", "preproject", "final_velocity", "pressure", "divergence")]
 
 
 class CfsError(RuntimeError):

@@ -557,17 +557,6 @@ int build_basis(const uint8_t* const bufs[3], const size_t sizes[3], cfs_basis**
   return CFS_OK;
 }
 
-struct DeviceGuard {
-  int prev = -1;
-  explicit DeviceGuard(int dev) {
-    if (hipGetDevice(&prev) == hipSuccess && prev != dev) (void)hipSetDevice(dev);
-    else prev = -1;
-  }
-  ~DeviceGuard() {
-    if (prev >= 0) (void)hipSetDevice(prev);
-  }
-};
-
 int read_file(const char* path, std::vector<uint8_t>* buf) {
   if (!path) return fail(CFS_ERR_ARG, "null path");
   FILE* fp = std::fopen(path, "rb");

@@ -21,4 +21,16 @@ const float* basis_damp(const cfs_basis* b);
 const void* basis_tables(const cfs_basis* b);
 int basis_num_blocks(const cfs_basis* b);
 int64_t basis_total_tokens(const cfs_basis* b);
+
+// Makes `dev` the calling thread's current device for the guard's lifetime.
+struct DeviceGuard {
+  int prev = -1;
+  explicit DeviceGuard(int dev) {
+    if (hipGetDevice(&prev) == hipSuccess && prev != dev) (void)hipSetDevice(dev);
+    else prev = -1;
+  }
+  ~DeviceGuard() {
+    if (prev >= 0) (void)hipSetDevice(prev);
+  }
+};
 }  // namespace cfs

@@ -60,17 +60,6 @@ struct cfs_batch {
 
 namespace {
 
-struct BatchDeviceGuard {
-  int prev = -1;
-  explicit BatchDeviceGuard(int dev) {
-    if (hipGetDevice(&prev) == hipSuccess && prev != dev) (void)hipSetDevice(dev);
-    else prev = -1;
-  }
-  ~BatchDeviceGuard() {
-    if (prev >= 0) (void)hipSetDevice(prev);
-  }
-};
-
 // Splits `grid` workgroups over the entries in proportion to
 // tokens + item_weight * items (each entry gets at least one) and assigns the
 // partial-q rows.  Returns false if the grid is smaller than the entry count.
@@ -158,7 +147,7 @@ int cfs_batch_create(cfs_basis* const* bases, int n, cfs_batch** out) {
     max_r = std::max(max_r, D.r);
     lds = std::max(lds, step_lds_bytes_for(D.r, D.nk));
   }
-  BatchDeviceGuard g(b->device);
+  cfs::DeviceGuard g(b->device);
   b->lds = lds;
   hipError_t e = hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, b->device);
   if (e != hipSuccess) return hip_fail(e, "hipDeviceGetAttribute");
@@ -195,7 +184,7 @@ int cfs_batch_bind(cfs_batch* b, const float* const* v_in, double* const* q_out,
     b->entries[i].q_in = q_in[i];
     b->entries[i].field = field[i];
   }
-  BatchDeviceGuard g(b->device);
+  cfs::DeviceGuard g(b->device);
   b->bound = true;
   b->drop_graph();
   return upload(b);
@@ -204,7 +193,7 @@ int cfs_batch_bind(cfs_batch* b, const float* const* v_in, double* const* q_out,
 int cfs_batch_step(cfs_batch* b, void* stream) {
   if (!b) return fail(CFS_ERR_ARG, "null batch");
   if (!b->bound) return fail(CFS_ERR_ARG, "cfs_batch_bind must precede cfs_batch_step");
-  BatchDeviceGuard g(b->device);
+  cfs::DeviceGuard g(b->device);
   hipError_t e = launch_batch_step(b->d_entries, (int)b->entries.size(), 3, b->grid, b->qrows, b->lds,
                                    (hipStream_t)stream);
   return e == hipSuccess ? CFS_OK : hip_fail(e, "batch step launch");
@@ -214,7 +203,7 @@ int cfs_batch_tune(cfs_batch* b, int reps, void* stream) {
   if (!b) return fail(CFS_ERR_ARG, "null batch");
   if (!b->bound) return fail(CFS_ERR_ARG, "cfs_batch_bind must precede cfs_batch_tune");
   if (reps < 1) reps = 3;
-  BatchDeviceGuard g(b->device);
+  cfs::DeviceGuard g(b->device);
   hipStream_t s = (hipStream_t)stream;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   hipError_t e = hipEventCreate(&e0);
@@ -266,7 +255,7 @@ int cfs_batch_get_config(const cfs_batch* b, int* grid, int* wg_per_cu, float* i
 int cfs_batch_set_config(cfs_batch* b, int wg_per_cu, float item_weight) {
   if (!b) return fail(CFS_ERR_ARG, "null batch");
   if (wg_per_cu < 1 || item_weight < 0) return fail(CFS_ERR_ARG, "bad batch configuration");
-  BatchDeviceGuard g(b->device);
+  cfs::DeviceGuard g(b->device);
   return apply(b, wg_per_cu, item_weight);
 }
 
@@ -274,7 +263,7 @@ int cfs_batch_graph_capture(cfs_batch* b, void* stream) {
   if (!b) return fail(CFS_ERR_ARG, "null batch");
   if (!b->bound) return fail(CFS_ERR_ARG, "cfs_batch_bind must precede cfs_batch_graph_capture");
   if (!stream) return fail(CFS_ERR_ARG, "graph capture needs a non-default stream");
-  BatchDeviceGuard g(b->device);
+  cfs::DeviceGuard g(b->device);
   b->drop_graph();
   hipStream_t s = (hipStream_t)stream;
   hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
@@ -291,7 +280,7 @@ int cfs_batch_graph_capture(cfs_batch* b, void* stream) {
 int cfs_batch_graph_launch(cfs_batch* b, void* stream) {
   if (!b) return fail(CFS_ERR_ARG, "null batch");
   if (!b->exec) return fail(CFS_ERR_ARG, "no captured graph (cfs_batch_graph_capture)");
-  BatchDeviceGuard g(b->device);
+  cfs::DeviceGuard g(b->device);
   hipError_t e = hipGraphLaunch(b->exec, (hipStream_t)stream);
   return e == hipSuccess ? CFS_OK : hip_fail(e, "hipGraphLaunch");
 }

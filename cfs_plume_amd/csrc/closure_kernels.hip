@@ -5,7 +5,8 @@
 // 1781-1854, approximates with key cells): q' = U^T A(U q), where A is the
 // first-order semi-Lagrangian self-advection of the full-grid velocity field
 // (VECTOR3_FIELD_3D::advect, VECTOR3_FIELD_3D.cpp:2001-2077: backtrace by
-// dt * v, clamp to [1.5, res - 2.5], trilinear interpolation).
+// dt * v, clamp to [1.5, res - 2.5], trilinear interpolation; the per-cell
+// kernels are in grid_ops.h).
 //
 // Learned closure: an MLP r -> H -> H -> H -> r (GELU after each hidden
 // layer) predicting the advection increment A(q) - q in scaled units.
@@ -20,69 +21,11 @@
 #include <algorithm>
 #include <cstdint>
 
+#include "grid_ops.h"
 #include "mfma_idct.h"
 
 namespace cfs {
 
-// ---------------------------------------------------------------------------
-// semi-Lagrangian self-advection of a full-grid interleaved vector field
-// ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) advect_kernel(const float* __restrict__ vel, float* __restrict__ out, int X,
-                                                     int Y, int Z, float dt0) {
-  const int n = X * Y * Z;
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= n) return;
-  const int x = idx % X, y = (idx / X) % Y, z = idx / (X * Y);
-  float xt = (float)x - dt0 * vel[3 * idx];
-  float yt = (float)y - dt0 * vel[3 * idx + 1];
-  float zt = (float)z - dt0 * vel[3 * idx + 2];
-  xt = fminf(fmaxf(xt, 1.5f), (float)X - 2.5f);
-  yt = fminf(fmaxf(yt, 1.5f), (float)Y - 2.5f);
-  zt = fminf(fmaxf(zt, 1.5f), (float)Z - 2.5f);
-  const int x0 = (int)xt, y0 = (int)yt, z0 = (int)zt;
-  const float s1 = xt - (float)x0, s0 = 1.0f - s1;
-  const float t1 = yt - (float)y0, t0 = 1.0f - t1;
-  const float u1 = zt - (float)z0, u0 = 1.0f - u1;
-  const int slab = X * Y;
-  const int i000 = x0 + y0 * X + z0 * slab;
-  const int i010 = i000 + X, i100 = i000 + 1, i110 = i000 + X + 1;
-  const int i001 = i000 + slab, i011 = i010 + slab, i101 = i100 + slab, i111 = i110 + slab;
-#pragma unroll
-  for (int c = 0; c < 3; c++) {
-    const float* f = vel + c;
-    out[3 * idx + c] = u0 * (s0 * (t0 * f[3 * i000] + t1 * f[3 * i010]) + s1 * (t0 * f[3 * i100] + t1 * f[3 * i110])) +
-                       u1 * (s0 * (t0 * f[3 * i001] + t1 * f[3 * i011]) + s1 * (t0 * f[3 * i101] + t1 * f[3 * i111]));
-  }
-}
-
-// semi-Lagrangian advection of a full-grid scalar field (density, heat) by
-// an interleaved velocity field (VECTOR3_FIELD_3D::advect for a FIELD_3D,
-// VECTOR3_FIELD_3D.cpp:2082-2160): the same backtrace and clamp as above,
-// one trilinear sample per cell
-__global__ void __launch_bounds__(256) advect_scalar_kernel(const float* __restrict__ vel,
-                                                            const float* __restrict__ old, float* __restrict__ out,
-                                                            int X, int Y, int Z, float dt0) {
-  const int n = X * Y * Z;
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= n) return;
-  const int x = idx % X, y = (idx / X) % Y, z = idx / (X * Y);
-  float xt = (float)x - dt0 * vel[3 * idx];
-  float yt = (float)y - dt0 * vel[3 * idx + 1];
-  float zt = (float)z - dt0 * vel[3 * idx + 2];
-  xt = fminf(fmaxf(xt, 1.5f), (float)X - 2.5f);
-  yt = fminf(fmaxf(yt, 1.5f), (float)Y - 2.5f);
-  zt = fminf(fmaxf(zt, 1.5f), (float)Z - 2.5f);
-  const int x0 = (int)xt, y0 = (int)yt, z0 = (int)zt;
-  const float s1 = xt - (float)x0, s0 = 1.0f - s1;
-  const float t1 = yt - (float)y0, t0 = 1.0f - t1;
-  const float u1 = zt - (float)z0, u0 = 1.0f - u1;
-  const int slab = X * Y;
-  const float* f = old + x0 + y0 * X + z0 * slab;
-  out[idx] = u0 * (s0 * (t0 * f[0] + t1 * f[X]) + s1 * (t0 * f[1] + t1 * f[X + 1])) +
-             u1 * (s0 * (t0 * f[slab] + t1 * f[slab + X]) + s1 * (t0 * f[slab + 1] + t1 * f[slab + X + 1]));
-}
-
-// q_out = alpha * q_out + forcing (forcing may be null)
 // Clears n 32-bit words.  Used instead of hipMemsetAsync on the per-step
 // paths, so that a caller who captures them in a HIP graph records kernel
 // nodes only.
@@ -90,6 +33,7 @@ __global__ void __launch_bounds__(256) zero_words_kernel(uint32_t* __restrict__ 
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) p[i] = 0u;
 }
 
+// q_out = alpha * q_out + forcing (forcing may be null)
 __global__ void axpb_kernel(double* __restrict__ q, int r, double alpha, const double* __restrict__ forcing) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j < r) q[j] = alpha * q[j] + (forcing ? forcing[j] : 0.0);
@@ -359,14 +303,15 @@ __global__ void __launch_bounds__(CL_H * KS) closure_step_kernel(ClosureNet net,
 // ---------------------------------------------------------------------------
 hipError_t launch_advect(const float* vel, float* out, int X, int Y, int Z, float dt0, hipStream_t s) {
   const int n = X * Y * Z;
-  hipLaunchKernelGGL(advect_kernel, dim3((n + 255) / 256), dim3(256), 0, s, vel, out, X, Y, Z, dt0);
+  hipLaunchKernelGGL(advect_vector_kernel<float>, dim3(grid_blocks(n)), dim3(GRID_TPB), 0, s, vel, out, X, Y, Z, dt0);
   return hipGetLastError();
 }
 
 hipError_t launch_advect_scalar(const float* vel, const float* old, float* out, int X, int Y, int Z, float dt0,
                                 hipStream_t s) {
   const int n = X * Y * Z;
-  hipLaunchKernelGGL(advect_scalar_kernel, dim3((n + 255) / 256), dim3(256), 0, s, vel, old, out, X, Y, Z, dt0);
+  hipLaunchKernelGGL(advect_scalar_field_kernel<float>, dim3(grid_blocks(n)), dim3(GRID_TPB), 0, s, vel, old, out, X,
+                     Y, Z, dt0);
   return hipGetLastError();
 }
 

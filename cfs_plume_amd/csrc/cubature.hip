@@ -168,17 +168,6 @@ struct cfs_cubature {
 
 namespace {
 
-struct DeviceSwitch {
-  int prev = -1;
-  explicit DeviceSwitch(int dev) {
-    if (hipGetDevice(&prev) == hipSuccess && prev != dev) (void)hipSetDevice(dev);
-    else prev = -1;
-  }
-  ~DeviceSwitch() {
-    if (prev >= 0) (void)hipSetDevice(prev);
-  }
-};
-
 #define CUB_HIP(call, what)                         \
   do {                                              \
     const hipError_t e_ = (call);                   \
@@ -206,7 +195,7 @@ int cfs_cubature_create(int K, const int32_t* cells, const double* weights, cfs_
   }
   std::unique_ptr<cfs_cubature> C(new cfs_cubature);
   C->device = cfs::basis_device(after);
-  DeviceSwitch g(C->device);
+  cfs::DeviceGuard g(C->device);
   C->K = K;
   C->r = info.num_cols;
   C->X = info.dims[0];
@@ -302,7 +291,7 @@ int cfs_cubature_advect(cfs_cubature* c, cfs_basis* pre, const double* q_in, dou
   if (st) return st;
   if (info.dims[0] != c->X || info.dims[1] != c->Y || info.dims[2] != c->Z)
     return cfs::fail(CFS_ERR_ARG, "the basis grid differs from the cubature's");
-  DeviceSwitch g(c->device);
+  cfs::DeviceGuard g(c->device);
   const hipStream_t s = (hipStream_t)stream;
   if ((st = cfs_unproject(pre, q_in, field, stream))) return st;
   hipLaunchKernelGGL(cubature_kernel, dim3(c->nwg, (c->r + CUB_TPB - 1) / CUB_TPB), dim3(CUB_TPB), 0, s, field, c->X,
@@ -316,7 +305,7 @@ int cfs_cubature_advect(cfs_cubature* c, cfs_basis* pre, const double* q_in, dou
 
 int cfs_cubature_destroy(cfs_cubature* c) {
   if (!c) return cfs::fail(CFS_ERR_ARG, "null cubature");
-  DeviceSwitch g(c->device);
+  cfs::DeviceGuard g(c->device);
   delete c;
   return CFS_OK;
 }

@@ -51,17 +51,6 @@ int scratch_for_device(int dev, float** out) {
   return CFS_OK;
 }
 
-struct DeviceSwitch {
-  int prev = -1;
-  explicit DeviceSwitch(int dev) {
-    if (hipGetDevice(&prev) == hipSuccess && prev != dev) (void)hipSetDevice(dev);
-    else prev = -1;
-  }
-  ~DeviceSwitch() {
-    if (prev >= 0) (void)hipSetDevice(prev);
-  }
-};
-
 }  // namespace
 
 extern "C" {
@@ -78,7 +67,7 @@ int cfs_reduced_advect_noalloc(cfs_basis* U, const double* q_in, double* q_out, 
     return cfs::fail(CFS_ERR_ARG, "full grid of " + std::to_string(3 * X * Y * Z / 3) +
                                       " cells exceeds the no-allocation scratch (" +
                                       std::to_string(NOALLOC_FIELD_FLOATS / 3) + " cells per field)");
-  DeviceSwitch g(cfs::basis_device(U));
+  cfs::DeviceGuard g(cfs::basis_device(U));
   float* base = nullptr;
   if ((st = scratch_for_device(cfs::basis_device(U), &base))) return st;
   float* field_a = base;

@@ -33,6 +33,9 @@
 
 #include "../../include/cfs_plume.h"
 #include "basis_internal.h"
+#include "grid_ops.h"
+
+using namespace cfs;  // the shared grid stencils
 
 namespace cfs {
 hipError_t launch_advect_scalar(const float* vel, const float* old, float* out, int X, int Y, int Z, float dt0,
@@ -41,113 +44,8 @@ hipError_t launch_advect_scalar(const float* vel, const float* old, float* out, 
 
 namespace {
 
-constexpr int TPB = 256;
+constexpr int TPB = GRID_TPB;
 constexpr int MAX_R = 256;
-
-__device__ __forceinline__ bool on_border(int x, int y, int z, int X, int Y, int Z) {
-  return x == 0 || y == 0 || z == 0 || x == X - 1 || y == Y - 1 || z == Z - 1;
-}
-
-// setZeroBorder of the velocity, then velocity += dt * (buoyancy * heat) e_y on
-// every cell (addBuoyancy adds to the force of every cell; the step adds
-// dt * force).
-__global__ void __launch_bounds__(TPB) buoyancy_kernel(float* __restrict__ vel, const float* __restrict__ heat, int X,
-                                                       int Y, int Z, float dt, float beta) {
-  const int i = blockIdx.x * TPB + threadIdx.x;
-  if (i >= X * Y * Z) return;
-  const int x = i % X, y = (i / X) % Y, z = i / (X * Y);
-  float* v = vel + 3 * (size_t)i;
-  if (on_border(x, y, z, X, Y, Z)) {
-    v[0] = 0.0f;
-    v[1] = 0.0f;
-    v[2] = 0.0f;
-  }
-  v[1] += dt * (beta * heat[i]);
-}
-
-// Neighbours of an interior cell for the vorticity stencils: the cell itself
-// in place of a neighbour on the border layer, and the matching inverse
-// spacings (one-sided 1/dx, central 0.5/dx).
-struct Stencil {
-  int up, down, out, in, right, left;
-  float dy, dz, dx;
-};
-__device__ __forceinline__ Stencil stencil(int i, int x, int y, int z, int X, int Y, int Z, float inv_dx) {
-  Stencil s;
-  const int slab = X * Y;
-  const float grid = 0.5f * inv_dx;
-  s.up = y == Y - 2 ? i : i + X;
-  s.down = y == 1 ? i : i - X;
-  s.dy = (s.up == i || s.down == i) ? inv_dx : grid;
-  s.out = z == Z - 2 ? i : i + slab;
-  s.in = z == 1 ? i : i - slab;
-  s.dz = (s.out == i || s.in == i) ? inv_dx : grid;
-  s.right = x == X - 2 ? i : i + 1;
-  s.left = x == 1 ? i : i - 1;
-  s.dx = (s.left == i || s.right == i) ? inv_dx : grid;
-  return s;
-}
-
-// the curl w of the velocity and |w| at interior cells (border entries stay 0)
-__global__ void __launch_bounds__(TPB) curl_kernel(const float* __restrict__ vel, float* __restrict__ w,
-                                                   float* __restrict__ wmag, int X, int Y, int Z, float inv_dx) {
-  const int i = blockIdx.x * TPB + threadIdx.x;
-  if (i >= X * Y * Z) return;
-  const int x = i % X, y = (i / X) % Y, z = i / (X * Y);
-  if (on_border(x, y, z, X, Y, Z)) return;
-  const Stencil s = stencil(i, x, y, z, X, Y, Z, inv_dx);
-  const float* v = vel;
-  const float w0 = (v[3 * s.up + 2] - v[3 * s.down + 2]) * s.dy + (-v[3 * s.out + 1] + v[3 * s.in + 1]) * s.dz;
-  const float w1 = (v[3 * s.out + 0] - v[3 * s.in + 0]) * s.dz + (-v[3 * s.right + 2] + v[3 * s.left + 2]) * s.dx;
-  const float w2 = (v[3 * s.right + 1] - v[3 * s.left + 1]) * s.dx + (-v[3 * s.up + 0] + v[3 * s.down + 0]) * s.dy;
-  w[3 * (size_t)i] = w0;
-  w[3 * (size_t)i + 1] = w1;
-  w[3 * (size_t)i + 2] = w2;
-  wmag[i] = sqrtf(w0 * w0 + w1 * w1 + w2 * w2);
-}
-
-// velocity += dt * eps * dx * (N x w) at interior cells, N = grad|w| / |grad|w||
-__global__ void __launch_bounds__(TPB) confinement_kernel(float* __restrict__ vel, const float* __restrict__ w,
-                                                          const float* __restrict__ wmag, int X, int Y, int Z,
-                                                          float inv_dx, float dt, float eps_dx) {
-  const int i = blockIdx.x * TPB + threadIdx.x;
-  if (i >= X * Y * Z) return;
-  const int x = i % X, y = (i / X) % Y, z = i / (X * Y);
-  if (on_border(x, y, z, X, Y, Z)) return;
-  const Stencil s = stencil(i, x, y, z, X, Y, Z, inv_dx);
-  float n0 = (wmag[s.right] - wmag[s.left]) * s.dx;
-  float n1 = (wmag[s.up] - wmag[s.down]) * s.dy;
-  float n2 = (wmag[s.out] - wmag[s.in]) * s.dz;
-  float m = sqrtf(n0 * n0 + n1 * n1 + n2 * n2);
-  if (m > 0.0f) {
-    m = 1.0f / m;
-    n0 *= m;
-    n1 *= m;
-    n2 *= m;
-    const float w0 = w[3 * (size_t)i], w1 = w[3 * (size_t)i + 1], w2 = w[3 * (size_t)i + 2];
-    float* v = vel + 3 * (size_t)i;
-    v[0] += dt * ((n1 * w2 - n2 * w1) * eps_dx);
-    v[1] += dt * (-(n0 * w2 - n2 * w0) * eps_dx);
-    v[2] += dt * ((n0 * w1 - n1 * w0) * eps_dx);
-  }
-}
-
-// The velocity border before the scalar advection: per axis, a wall zeroes
-// the normal component on its two faces (setZeroX/Y/Z), an open side copies
-// it from the neighbouring layer (copyBorderX/Y/Z).
-__global__ void __launch_bounds__(TPB) velocity_border_kernel(float* __restrict__ vel, int X, int Y, int Z,
-                                                              int open_x, int open_y, int open_z) {
-  const int i = blockIdx.x * TPB + threadIdx.x;
-  if (i >= X * Y * Z) return;
-  const int x = i % X, y = (i / X) % Y, z = i / (X * Y);
-  float* v = vel + 3 * (size_t)i;
-  if (x == 0 || x == X - 1) v[0] = open_x ? vel[3 * (size_t)(x == 0 ? i + 1 : i - 1)] : 0.0f;
-  if (y == 0 || y == Y - 1) v[1] = open_y ? vel[3 * (size_t)(y == 0 ? i + X : i - X) + 1] : 0.0f;
-  if (z == 0 || z == Z - 1) {
-    const int slab = X * Y;
-    v[2] = open_z ? vel[3 * (size_t)(z == 0 ? i + slab : i - slab) + 2] : 0.0f;
-  }
-}
 
 __global__ void __launch_bounds__(TPB) copy2_kernel(const float* __restrict__ a, float* __restrict__ a_old,
                                                     const float* __restrict__ b, float* __restrict__ b_old, int n) {
@@ -155,39 +53,6 @@ __global__ void __launch_bounds__(TPB) copy2_kernel(const float* __restrict__ a,
   if (i >= n) return;
   a_old[i] = a[i];
   b_old[i] = b[i];
-}
-
-// FIELD_3D::setZeroBorder of density and heat
-__global__ void __launch_bounds__(TPB) zero_border2_kernel(float* __restrict__ a, float* __restrict__ b, int X, int Y,
-                                                           int Z) {
-  const int i = blockIdx.x * TPB + threadIdx.x;
-  if (i >= X * Y * Z) return;
-  const int x = i % X, y = (i / X) % Y, z = i / (X * Y);
-  if (on_border(x, y, z, X, Y, Z)) {
-    a[i] = 0.0f;
-    b[i] = 0.0f;
-  }
-}
-
-// addSmokeTestCase (FLUID_3D_MIC.cpp:674-703) on density and heat: 1 inside
-// the column of radius 0.075 xTotal around (0.4 xTotal, 0.5 zTotal) for
-// y in [(int)(0.05 Y), (int)(0.10 Y)], lengths in units of dx = 1 / maxRes.
-__global__ void __launch_bounds__(TPB) smoke_kernel(float* __restrict__ dens, float* __restrict__ heat, int X, int Y,
-                                                    int Z, int y0, int y1) {
-  const int i = blockIdx.x * TPB + threadIdx.x;
-  if (i >= X * Y * Z) return;
-  const int x = i % X, y = (i / X) % Y, z = i / (X * Y);
-  if (y < y0 || y > y1) return;
-  const int maxres = max(X, max(Y, Z));
-  const double dx = 1.0 / (double)(float)maxres;
-  const double xtot = dx * X, ztot = dx * Z;
-  const double xl = x * dx - xtot * (double)0.4f;
-  const double zl = z * dx - ztot * (double)0.5f;
-  const float radius = sqrtf((float)(xl * xl + zl * zl));
-  if ((double)radius < (double)0.075f * xtot) {
-    dens[i] = 1.0f;
-    heat[i] = 1.0f;
-  }
 }
 
 // q <- D q, then q <- P q + I q (FP64), r <= 256, one workgroup.  q_in and
@@ -235,17 +100,6 @@ struct cfs_plume {
 };
 
 namespace {
-
-struct DeviceSwitch {
-  int prev = -1;
-  explicit DeviceSwitch(int dev) {
-    if (hipGetDevice(&prev) == hipSuccess && prev != dev) (void)hipSetDevice(dev);
-    else prev = -1;
-  }
-  ~DeviceSwitch() {
-    if (prev >= 0) (void)hipSetDevice(prev);
-  }
-};
 
 template <class T>
 int dev_alloc(cfs_plume* p, T** out, size_t count, const T* host) {
@@ -299,7 +153,7 @@ int cfs_plume_create(cfs_basis* pre, cfs_basis* fin, const double* damping, cons
   if (!(prm->dt > 0.0) || !(prm->dx > 0.0)) return cfs::fail(CFS_ERR_ARG, "dt and dx must be positive");
   std::unique_ptr<cfs_plume> P(new cfs_plume);
   P->device = cfs::basis_device(pre);
-  DeviceSwitch g(P->device);
+  cfs::DeviceGuard g(P->device);
   P->pre = pre;
   P->fin = fin;
   P->prm = *prm;
@@ -348,10 +202,10 @@ int cfs_plume_set_advection(cfs_plume* p, int mode, cfs_cubature* cub, cfs_closu
 
 int cfs_plume_add_smoke(cfs_plume* p, float* density, float* heat, void* stream) {
   if (!p || !density || !heat) return cfs::fail(CFS_ERR_ARG, "null argument");
-  DeviceSwitch g(p->device);
+  cfs::DeviceGuard g(p->device);
   const int n = p->X * p->Y * p->Z;
   const int y0 = (int)(0.05 * p->Y), y1 = (int)(0.10 * p->Y);
-  hipLaunchKernelGGL(smoke_kernel, dim3((n + TPB - 1) / TPB), dim3(TPB), 0, (hipStream_t)stream, density, heat, p->X,
+  hipLaunchKernelGGL(smoke_kernel<float>, dim3((n + TPB - 1) / TPB), dim3(TPB), 0, (hipStream_t)stream, density, heat, p->X,
                      p->Y, p->Z, y0, y1);
   PL_HIP(hipGetLastError(), "smoke launch");
   return CFS_OK;
@@ -359,7 +213,7 @@ int cfs_plume_add_smoke(cfs_plume* p, float* density, float* heat, void* stream)
 
 int cfs_plume_step(cfs_plume* p, float* velocity, float* density, float* heat, double* q, void* stream) {
   if (!p || !velocity || !density || !heat || !q) return cfs::fail(CFS_ERR_ARG, "null argument");
-  DeviceSwitch g(p->device);
+  cfs::DeviceGuard g(p->device);
   const hipStream_t s = (hipStream_t)stream;
   const int X = p->X, Y = p->Y, Z = p->Z, n = X * Y * Z;
   const dim3 grid((n + TPB - 1) / TPB), block(TPB);
@@ -367,26 +221,26 @@ int cfs_plume_step(cfs_plume* p, float* velocity, float* density, float* heat, d
   const float dt0 = (float)(p->prm.dt / p->prm.dx);
   int st;
   // forces
-  hipLaunchKernelGGL(buoyancy_kernel, grid, block, 0, s, velocity, heat, X, Y, Z, dt, (float)p->prm.buoyancy);
+  hipLaunchKernelGGL(buoyancy_kernel<float>, grid, block, 0, s, velocity, heat, X, Y, Z, dt, (float)p->prm.buoyancy);
   PL_HIP(hipGetLastError(), "buoyancy launch");
   if (p->prm.vorticity_eps > 0.0) {
-    hipLaunchKernelGGL(curl_kernel, grid, block, 0, s, velocity, p->d_w, p->d_wmag, X, Y, Z, inv_dx);
+    hipLaunchKernelGGL(curl_kernel<float>, grid, block, 0, s, velocity, p->d_w, p->d_wmag, X, Y, Z, inv_dx);
     PL_HIP(hipGetLastError(), "curl launch");
-    hipLaunchKernelGGL(confinement_kernel, grid, block, 0, s, velocity, p->d_w, p->d_wmag, X, Y, Z, inv_dx, dt,
+    hipLaunchKernelGGL(confinement_kernel<float>, grid, block, 0, s, velocity, p->d_w, p->d_wmag, X, Y, Z, inv_dx, dt,
                        (float)(p->prm.vorticity_eps * p->prm.dx));
     PL_HIP(hipGetLastError(), "confinement launch");
   }
   // projection into the subspace
   if ((st = cfs_project(p->pre, velocity, q, stream))) return st;
   // density and heat
-  hipLaunchKernelGGL(velocity_border_kernel, grid, block, 0, s, velocity, X, Y, Z, p->prm.open_x, p->prm.open_y,
-                     p->prm.open_z);
+  hipLaunchKernelGGL(velocity_border_kernel<float>, grid, block, 0, s, velocity, X, Y, Z, p->prm.open_x,
+                     p->prm.open_y, p->prm.open_z, 1);
   PL_HIP(hipGetLastError(), "border launch");
   hipLaunchKernelGGL(copy2_kernel, grid, block, 0, s, density, p->d_dold, heat, p->d_hold, n);
   PL_HIP(hipGetLastError(), "copy launch");
   PL_HIP(cfs::launch_advect_scalar(velocity, p->d_dold, density, X, Y, Z, dt0, s), "advect launch");
   PL_HIP(cfs::launch_advect_scalar(velocity, p->d_hold, heat, X, Y, Z, dt0, s), "advect launch");
-  hipLaunchKernelGGL(zero_border2_kernel, grid, block, 0, s, density, heat, X, Y, Z);
+  hipLaunchKernelGGL(zero_border2_kernel<float>, grid, block, 0, s, density, heat, X, Y, Z);
   PL_HIP(hipGetLastError(), "border launch");
   // advection in the subspace
   if (p->mode == CFS_PLUME_ADVECT_CUBATURE)
@@ -405,7 +259,7 @@ int cfs_plume_step(cfs_plume* p, float* velocity, float* density, float* heat, d
 
 int cfs_plume_destroy(cfs_plume* p) {
   if (!p) return cfs::fail(CFS_ERR_ARG, "null plume");
-  DeviceSwitch g(p->device);
+  cfs::DeviceGuard g(p->device);
   delete p;
   return CFS_OK;
 }

@@ -110,6 +110,18 @@ class DeviceArray:
         return d
 
     @classmethod
+    def view(cls, ptr: int, shape, dtype, owner=None) -> "DeviceArray":
+        """A non-owning view of device memory held by `owner` (kept alive)."""
+        d = cls.__new__(cls)
+        d.shape = tuple(int(s) for s in np.atleast_1d(shape)) if not isinstance(shape, int) else (int(shape),)
+        d.dtype = np.dtype(dtype)
+        d.nbytes = int(np.prod(d.shape)) * d.dtype.itemsize
+        d.ptr = ptr
+        d._view = True
+        d._owner = owner
+        return d
+
+    @classmethod
     def zeros(cls, shape, dtype) -> "DeviceArray":
         d = cls(shape, dtype)
         d.fill_zero()
@@ -129,6 +141,9 @@ class DeviceArray:
         check(hip().hipMemset(self.ptr, 0, self.nbytes), "hipMemset")
 
     def free(self):
+        if getattr(self, "_view", False):  # the owner frees the memory
+            self.ptr = None
+            return
         if getattr(self, "ptr", None):
             hip().hipFree(self.ptr)
             self.ptr = None

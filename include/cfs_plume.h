@@ -295,6 +295,50 @@ int cfs_plume_add_smoke(cfs_plume* p, float* density, float* heat, void* stream)
 int cfs_plume_step(cfs_plume* p, float* velocity, float* density, float* heat, double* q, void* stream);
 int cfs_plume_destroy(cfs_plume* p);
 
+/* ---- the full-rank plume (snapshot source of the reduced bases) --------- */
+/* FLUID_3D_MIC (src/integrators/FLUID_3D_MIC.cpp) on the device in FP64:
+ * cfs_fluid_add_smoke = addSmokeColumn (:655-662), cfs_fluid_step = step
+ * (:179-233: forces, advectStam :595-653, the peeled damping matrix
+ * :1251-1307, project :314-402 with the MIC-preconditioned CG of
+ * solvePoisson :918-1017), the snapshots of appendStreams (:1329-1480).
+ * Fields live in the solver (cfs_fluid_fields: full grids X x Y x Z, x
+ * fastest, velocity interleaved xyz, device FP64). */
+typedef struct cfs_fluid cfs_fluid;
+typedef struct {
+  double dt;             /* _dt */
+  double dx;             /* _dx = 1 / max resolution */
+  double buoyancy;       /* _buoyancy */
+  double vorticity_eps;  /* _vorticityEps (the drivers set it from the cfg "vorticity", default 0) */
+  double damping_w;      /* w of buildPeeledDampingMatrixFull */
+  double solver_eps;     /* _solverEps: CG stops when the residual maximum is at most this */
+  int iterations;        /* _iterations: CG iteration cap */
+  int boundaries[6];     /* front, back (z), left, right (x), top, bottom (y): 1 wall, 0 open
+                            (FLUID_3D_MIC.cpp:100-118) */
+} cfs_fluid_params;
+/* Device pointers (any may be null) that cfs_fluid_step fills with the
+ * peeled fields of appendStreams: velocity before advection, before
+ * diffusion, before projection, final (3 (X-2)(Y-2)(Z-2) doubles each), the
+ * pressure and the divergence ((X-2)(Y-2)(Z-2) each). */
+typedef struct {
+  double* preadvect;
+  double* prediffuse;
+  double* preproject;
+  double* final_velocity;
+  double* pressure;
+  double* divergence;
+} cfs_fluid_snapshots;
+/* The FLUID_3D_MIC constructor's constants (FLUID_3D_MIC.cpp:29-56). */
+void cfs_fluid_default_params(int X, int Y, int Z, cfs_fluid_params* params);
+int cfs_fluid_create(int X, int Y, int Z, const cfs_fluid_params* params, cfs_fluid** out);
+int cfs_fluid_fields(cfs_fluid* f, double** velocity, double** density, double** heat, double** pressure);
+int cfs_fluid_add_smoke(cfs_fluid* f, void* stream);
+/* One step; synchronises `stream` once per CG iteration (the CG control
+ * flow runs on the host). */
+int cfs_fluid_step(cfs_fluid* f, const cfs_fluid_snapshots* snapshots, void* stream);
+/* CG iterations and residual maximum of the last pressure solve. */
+int cfs_fluid_last_solve(const cfs_fluid* f, int* iterations, double* residual_max);
+int cfs_fluid_destroy(cfs_fluid* f);
+
 /* ---- encoder (host) ---------------------------------------------------------
  * CompressAndWriteMatrixComponents (COMPRESSION.cpp:2271-2359) with the SVD
  * transform off, as on the live path (:2312).  U is FP64 column-major:
