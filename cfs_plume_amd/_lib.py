@@ -47,11 +47,13 @@ SIGNATURES = {
     "cfs_reduced_advect": (_i, [_vp, _vp, _vp, _vp, _vp, ctypes.c_float, ctypes.c_double, _vp, _vp]),
     "cfs_reduced_advect_noalloc": (_i, [_vp, _vp, _vp, ctypes.c_float, ctypes.c_double, _vp, _vp]),
     "cfs_cubature_create": (_i, [_i, _vp, _vp, _vp, ctypes.POINTER(_vp)]),
+    "cfs_cubature_create_rows": (_i, [_i, _vp, _vp, _i, _i, _i, _i, _vp, ctypes.POINTER(_vp)]),
     "cfs_cubature_open_file": (_i, [_cp, _vp, ctypes.POINTER(_vp)]),
     "cfs_cubature_write_file": (_i, [_cp, _i, _vp, _vp]),
     "cfs_cubature_get_info": (_i, [_vp, ctypes.POINTER(_i), ctypes.POINTER(_i)]),
     "cfs_cubature_get_cells": (_i, [_vp, _vp, _vp]),
     "cfs_cubature_advect": (_i, [_vp, _vp, _vp, _vp, _vp, ctypes.c_float, ctypes.c_double, _vp, _vp]),
+    "cfs_cubature_advect_field": (_i, [_vp, _vp, _vp, ctypes.c_float, ctypes.c_double, _vp, _vp]),
     "cfs_cubature_destroy": (_i, [_vp]),
     "cfs_nnls": (_i, [_vp, _i, _i, _vp, _i, _vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_i)]),
     "cfs_advect_field": (_i, [_vp, _vp, _i, _i, _i, ctypes.c_float, _vp]),
@@ -70,7 +72,8 @@ SIGNATURES = {
     "cfs_closure_train_step": (_i, [_vp, _vp, _vp, _vp, _i, ctypes.c_float, _vp, _vp]),
     "cfs_closure_step": (_i, [_vp, _vp, _vp, _i, ctypes.c_double, _vp, _vp]),
     "cfs_plume_default_params": (None, [_i, _i, _i, _vp]),
-    "cfs_plume_create": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(_vp)]),
+    "cfs_plume_create": (_i, [_vp, _vp, _i, _i, _vp, _vp, _vp, _vp, ctypes.POINTER(_vp)]),
+    "cfs_plume_create_dense": (_i, [_vp, _i, _vp, _i, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, ctypes.POINTER(_vp)]),
     "cfs_plume_set_advection": (_i, [_vp, _i, _vp, _vp]),
     "cfs_plume_add_smoke": (_i, [_vp, _vp, _vp, _vp]),
     "cfs_plume_step": (_i, [_vp, _vp, _vp, _vp, _vp, _vp]),
@@ -83,6 +86,10 @@ SIGNATURES = {
     "cfs_fluid_step": (_i, [_vp, _vp, _vp]),
     "cfs_fluid_last_solve": (_i, [_vp, ctypes.POINTER(_i), ctypes.POINTER(ctypes.c_double)]),
     "cfs_fluid_destroy": (_i, [_vp]),
+    "cfs_gemm_tn": (_i, [_vp, ctypes.c_int64, _i, ctypes.c_int64, _vp, _i, ctypes.c_int64, _vp, _vp]),
+    "cfs_gemm_nn": (_i, [_vp, ctypes.c_int64, _i, ctypes.c_int64, _vp, _i, _vp, ctypes.c_int64, _vp]),
+    "cfs_apply_operator": (_i, [_i, _i, _i, _i, _vp, ctypes.c_double, ctypes.c_double, _vp, ctypes.c_int64, _i, _vp,
+                                ctypes.c_int64, _vp]),
     "cfs_encode_matrix": (_i, [_vp, _i, _i, _i, _i, _i, ctypes.c_double, _i, _i, ctypes.POINTER(_vp)]),
     "cfs_encode_block": (_i, [_vp, _i, ctypes.c_double, _i, ctypes.POINTER(ctypes.c_double),
                               ctypes.POINTER(ctypes.c_double), _vp, ctypes.POINTER(_i)]),

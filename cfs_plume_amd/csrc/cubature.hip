@@ -178,15 +178,22 @@ namespace {
 
 extern "C" {
 
-int cfs_cubature_create(int K, const int32_t* cells, const double* weights, cfs_basis* after, cfs_cubature** out) {
+}  // extern "C"
+
+namespace {
+
+// A cubature over the peeled grid dims[3] with `r` outputs; `fill` writes the
+// unscaled after-rows ([K][r][3], entry (c, j) of cell i at i 3r + c + 3j)
+// into the device array it is given; they are then scaled by the weights.
+template <typename Fill>
+int make_cubature(int K, const int32_t* cells, const double* weights, const int dims[3], int r, int device,
+                  Fill fill, cfs_cubature** out) {
   if (!out) return cfs::fail(CFS_ERR_ARG, "null output handle");
   *out = nullptr;
-  if (!cells || !weights || !after) return cfs::fail(CFS_ERR_ARG, "null argument");
+  if (!cells || !weights) return cfs::fail(CFS_ERR_ARG, "null argument");
   if (K < 1) return cfs::fail(CFS_ERR_ARG, "a cubature needs at least one key cell");
-  cfs_basis_info info;
-  int st = cfs_basis_get_info(after, &info);
-  if (st) return st;
-  const long ncell = (long)info.dims[0] * info.dims[1] * info.dims[2];
+  if (r < 1) return cfs::fail(CFS_ERR_ARG, "the after-basis needs at least one column");
+  const long ncell = (long)dims[0] * dims[1] * dims[2];
   for (int i = 0; i < K; i++) {
     if (cells[i] < 0 || cells[i] >= ncell)
       return cfs::fail(CFS_ERR_ARG, "key cell " + std::to_string(i) + " (" + std::to_string(cells[i]) +
@@ -194,13 +201,13 @@ int cfs_cubature_create(int K, const int32_t* cells, const double* weights, cfs_
     if (!std::isfinite(weights[i])) return cfs::fail(CFS_ERR_ARG, "key cell weight is not finite");
   }
   std::unique_ptr<cfs_cubature> C(new cfs_cubature);
-  C->device = cfs::basis_device(after);
+  C->device = device;
   cfs::DeviceGuard g(C->device);
   C->K = K;
-  C->r = info.num_cols;
-  C->X = info.dims[0];
-  C->Y = info.dims[1];
-  C->Z = info.dims[2];
+  C->r = r;
+  C->X = dims[0];
+  C->Y = dims[1];
+  C->Z = dims[2];
   C->nwg = (K + CUB_CELLS - 1) / CUB_CELLS;
   C->cells.assign(cells, cells + K);
   C->weights.assign(weights, weights + K);
@@ -209,10 +216,8 @@ int cfs_cubature_create(int K, const int32_t* cells, const double* weights, cfs_
   CUB_HIP(hipMalloc(&C->d_after, (size_t)K * per * 8), "hipMalloc");
   CUB_HIP(hipMalloc(&C->d_partials, (size_t)C->nwg * C->r * 8), "hipMalloc");
   CUB_HIP(hipMemcpy(C->d_cells, cells, (size_t)K * 4, hipMemcpyHostToDevice), "hipMemcpy");
-  // After_i = w_i U_after(x_i)^T: the 3 x r rows of each key cell
-  // (GetSubmatrixNoSVDSparse), column-major, entry (c, j) at c + 3 j
-  for (int i = 0; i < K; i++)
-    if ((st = cfs_get_submatrix(after, 3 * cells[i], C->d_after + (size_t)i * per, nullptr))) return st;
+  int st = fill(C->d_after);
+  if (st) return st;
   double* d_w = nullptr;
   CUB_HIP(hipMalloc(&d_w, (size_t)K * 8), "hipMalloc");
   hipError_t e = hipMemcpy(d_w, weights, (size_t)K * 8, hipMemcpyHostToDevice);
@@ -227,6 +232,42 @@ int cfs_cubature_create(int K, const int32_t* cells, const double* weights, cfs_
   if (e != hipSuccess) return cfs::hip_fail(e, "building the after-matrices");
   *out = C.release();
   return CFS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cfs_cubature_create(int K, const int32_t* cells, const double* weights, cfs_basis* after, cfs_cubature** out) {
+  if (!after) return cfs::fail(CFS_ERR_ARG, "null argument");
+  cfs_basis_info info;
+  int st = cfs_basis_get_info(after, &info);
+  if (st) return st;
+  // After_i = w_i U_after(x_i)^T: the 3 x r rows of each key cell
+  // (GetSubmatrixNoSVDSparse), column-major, entry (c, j) at c + 3 j
+  auto fill = [&](double* d_after) {
+    const size_t per = (size_t)3 * info.num_cols;
+    for (int i = 0; i < K; i++) {
+      const int s2 = cfs_get_submatrix(after, 3 * cells[i], d_after + (size_t)i * per, nullptr);
+      if (s2) return s2;
+    }
+    return (int)CFS_OK;
+  };
+  return make_cubature(K, cells, weights, info.dims, info.num_cols, cfs::basis_device(after), fill, out);
+}
+
+int cfs_cubature_create_rows(int K, const int32_t* cells, const double* weights, int X, int Y, int Z, int r,
+                             const double* rows, cfs_cubature** out) {
+  if (!rows) return cfs::fail(CFS_ERR_ARG, "null rows");
+  if (X < 1 || Y < 1 || Z < 1) return cfs::fail(CFS_ERR_ARG, "bad grid");
+  int device = 0;
+  CUB_HIP(hipGetDevice(&device), "hipGetDevice");
+  const int dims[3] = {X, Y, Z};
+  auto fill = [&](double* d_after) {
+    const hipError_t e = hipMemcpy(d_after, rows, (size_t)K * 3 * r * 8, hipMemcpyHostToDevice);
+    return e == hipSuccess ? (int)CFS_OK : cfs::hip_fail(e, "hipMemcpy");
+  };
+  return make_cubature(K, cells, weights, dims, r, device, fill, out);
 }
 
 int cfs_cubature_open_file(const char* path, cfs_basis* after, cfs_cubature** out) {
@@ -283,6 +324,20 @@ int cfs_cubature_get_cells(const cfs_cubature* c, int32_t* cells, double* weight
   return CFS_OK;
 }
 
+int cfs_cubature_advect_field(cfs_cubature* c, const float* field, double* q_out, float dt0, double alpha,
+                              const double* forcing, void* stream) {
+  if (!c || !field || !q_out) return cfs::fail(CFS_ERR_ARG, "null argument");
+  cfs::DeviceGuard g(c->device);
+  const hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(cubature_kernel, dim3(c->nwg, (c->r + CUB_TPB - 1) / CUB_TPB), dim3(CUB_TPB), 0, s, field, c->X,
+                     c->Y, c->Z, c->d_cells, c->K, c->d_after, c->r, (double)dt0, c->d_partials);
+  CUB_HIP(hipGetLastError(), "cubature launch");
+  hipLaunchKernelGGL(cubature_reduce_kernel, dim3((c->r + 255) / 256), dim3(256), 0, s, c->d_partials, c->nwg, c->r,
+                     alpha, forcing, q_out);
+  CUB_HIP(hipGetLastError(), "cubature reduce launch");
+  return CFS_OK;
+}
+
 int cfs_cubature_advect(cfs_cubature* c, cfs_basis* pre, const double* q_in, double* q_out, float* field, float dt0,
                         double alpha, const double* forcing, void* stream) {
   if (!c || !pre || !q_in || !q_out || !field) return cfs::fail(CFS_ERR_ARG, "null argument");
@@ -291,16 +346,8 @@ int cfs_cubature_advect(cfs_cubature* c, cfs_basis* pre, const double* q_in, dou
   if (st) return st;
   if (info.dims[0] != c->X || info.dims[1] != c->Y || info.dims[2] != c->Z)
     return cfs::fail(CFS_ERR_ARG, "the basis grid differs from the cubature's");
-  cfs::DeviceGuard g(c->device);
-  const hipStream_t s = (hipStream_t)stream;
   if ((st = cfs_unproject(pre, q_in, field, stream))) return st;
-  hipLaunchKernelGGL(cubature_kernel, dim3(c->nwg, (c->r + CUB_TPB - 1) / CUB_TPB), dim3(CUB_TPB), 0, s, field, c->X,
-                     c->Y, c->Z, c->d_cells, c->K, c->d_after, c->r, (double)dt0, c->d_partials);
-  CUB_HIP(hipGetLastError(), "cubature launch");
-  hipLaunchKernelGGL(cubature_reduce_kernel, dim3((c->r + 255) / 256), dim3(256), 0, s, c->d_partials, c->nwg, c->r,
-                     alpha, forcing, q_out);
-  CUB_HIP(hipGetLastError(), "cubature reduce launch");
-  return CFS_OK;
+  return cfs_cubature_advect_field(c, field, q_out, dt0, alpha, forcing, stream);
 }
 
 int cfs_cubature_destroy(cfs_cubature* c) {

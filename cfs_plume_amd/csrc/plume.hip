@@ -55,26 +55,48 @@ __global__ void __launch_bounds__(TPB) copy2_kernel(const float* __restrict__ a,
   b_old[i] = b[i];
 }
 
-// q <- D q, then q <- P q + I q (FP64), r <= 256, one workgroup.  q_in and
-// q_out may alias.
+// the peeled interior of an FP32 full-grid vector field as an FP64 column
+__global__ void __launch_bounds__(TPB) peel_to_column_kernel(const float* __restrict__ f, double* __restrict__ col, int X,
+                                                             int Y, int Z) {
+  const int Xp = X - 2, Yp = Y - 2, Zp = Z - 2;
+  const int j = blockIdx.x * TPB + threadIdx.x;
+  if (j >= Xp * Yp * Zp) return;
+  const int x = j % Xp + 1, y = (j / Xp) % Yp + 1, z = j / (Xp * Yp) + 1;
+  const size_t i = (size_t)x + (size_t)X * (y + (size_t)Y * z);
+  for (int c = 0; c < 3; c++) col[3 * (size_t)j + c] = (double)f[3 * i + c];
+}
+
+// setWithPeeled: an FP64 peeled column into the interior of an FP32 field
+__global__ void __launch_bounds__(TPB) column_to_interior_kernel(const double* __restrict__ col, float* __restrict__ f,
+                                                                 int X, int Y, int Z) {
+  const int Xp = X - 2, Yp = Y - 2, Zp = Z - 2;
+  const int j = blockIdx.x * TPB + threadIdx.x;
+  if (j >= Xp * Yp * Zp) return;
+  const int x = j % Xp + 1, y = (j / Xp) % Yp + 1, z = j / (Xp * Yp) + 1;
+  const size_t i = (size_t)x + (size_t)X * (y + (size_t)Y * z);
+  for (int c = 0; c < 3; c++) f[3 * i + c] = (float)col[3 * (size_t)j + c];
+}
+
+// q <- D q (r_proj x r_adv), then q <- P q + I q (r_fin x r_proj), FP64,
+// every dimension <= 256, one workgroup.  q_in and q_out may alias.
 __global__ void __launch_bounds__(MAX_R) reduced_ops_kernel(const double* __restrict__ D, const double* __restrict__ P,
                                                             const double* __restrict__ I, const double* q_in,
-                                                            double* q_out, int r) {
+                                                            double* q_out, int r_adv, int r_proj, int r_fin) {
   __shared__ double a[MAX_R], t[MAX_R];
   const int j = threadIdx.x;
-  if (j < r) a[j] = q_in[j];
+  if (j < r_adv) a[j] = q_in[j];
   __syncthreads();
-  if (j < r) {
+  if (j < r_proj) {
     double s = 0.0;
-    for (int k = 0; k < r; k++) s += D[(size_t)j * r + k] * a[k];
+    for (int k = 0; k < r_adv; k++) s += D[(size_t)j * r_adv + k] * a[k];
     t[j] = s;
   }
   __syncthreads();
-  if (j < r) {
+  if (j < r_fin) {
     double p = 0.0, c = 0.0;
-    for (int k = 0; k < r; k++) {
-      p += P[(size_t)j * r + k] * t[k];
-      c += I[(size_t)j * r + k] * t[k];
+    for (int k = 0; k < r_proj; k++) {
+      p += P[(size_t)j * r_proj + k] * t[k];
+      c += I[(size_t)j * r_proj + k] * t[k];
     }
     q_out[j] = p + c;
   }
@@ -87,11 +109,18 @@ struct cfs_plume {
   cfs_basis* pre = nullptr;
   cfs_basis* fin = nullptr;
   cfs_plume_params prm{};
-  int X = 0, Y = 0, Z = 0, r = 0;  // full grid, columns
+  int X = 0, Y = 0, Z = 0;                      // full grid
+  int r_pre = 0, r_adv = 0, r_proj = 0, r_fin = 0;  // columns of U_preadvect, the advected, preprojected
+                                                    // and final coordinates
   int mode = CFS_PLUME_ADVECT_EXACT;
   cfs_cubature* cub = nullptr;
   cfs_closure* closure = nullptr;
   double *d_D = nullptr, *d_P = nullptr, *d_I = nullptr, *d_q = nullptr;
+  // dense bases (the uncompressed solver): device FP64, 3 Npeel x r column-major,
+  // owned by the caller; null for compressed bases
+  const double* U_pre_dense = nullptr;
+  const double* U_fin_dense = nullptr;
+  double* d_col = nullptr;  // an FP64 peeled vector field
   float *d_w = nullptr, *d_wmag = nullptr, *d_dold = nullptr, *d_hold = nullptr, *d_fa = nullptr, *d_fb = nullptr;
   std::vector<void*> allocs;
   ~cfs_plume() {
@@ -118,6 +147,34 @@ int dev_alloc(cfs_plume* p, T** out, size_t count, const T* host) {
     if (e_ != hipSuccess) return cfs::hip_fail(e_, what);   \
   } while (0)
 
+long peeled(const cfs_plume* p) { return (long)(p->X - 2) * (p->Y - 2) * (p->Z - 2); }
+
+// q = U^T v with U_preadvect, compressed or dense
+int project_pre(cfs_plume* p, const float* field, double* q, void* stream) {
+  if (!p->U_pre_dense) return cfs_project(p->pre, field, q, stream);
+  const hipStream_t s = (hipStream_t)stream;
+  const long np = peeled(p);
+  hipLaunchKernelGGL(peel_to_column_kernel, dim3((unsigned)((np + TPB - 1) / TPB)), dim3(TPB), 0, s, field, p->d_col,
+                     p->X, p->Y, p->Z);
+  PL_HIP(hipGetLastError(), "peel launch");
+  return cfs_gemm_tn(p->U_pre_dense, 3 * np, p->r_pre, 3 * np, p->d_col, 1, 3 * np, q, stream);
+}
+
+// field interior = U q, with U_preadvect (which = 0) or U_final (which = 1)
+int unproject_with(cfs_plume* p, int which, const double* q, float* field, void* stream) {
+  const double* U = which ? p->U_fin_dense : p->U_pre_dense;
+  if (!U) return cfs_unproject(which ? p->fin : p->pre, q, field, stream);
+  const hipStream_t s = (hipStream_t)stream;
+  const long np = peeled(p);
+  const int r = which ? p->r_fin : p->r_pre;
+  int st = cfs_gemm_nn(U, 3 * np, r, 3 * np, q, 1, p->d_col, 3 * np, stream);
+  if (st) return st;
+  hipLaunchKernelGGL(column_to_interior_kernel, dim3((unsigned)((np + TPB - 1) / TPB)), dim3(TPB), 0, s, p->d_col,
+                     field, p->X, p->Y, p->Z);
+  PL_HIP(hipGetLastError(), "interior launch");
+  return CFS_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -138,40 +195,88 @@ void cfs_plume_default_params(int X, int Y, int Z, cfs_plume_params* prm) {
   prm->open_z = 0;
 }
 
-int cfs_plume_create(cfs_basis* pre, cfs_basis* fin, const double* damping, const double* ptof,
-                     const double* inverse_product, const cfs_plume_params* prm, cfs_plume** out) {
+}  // extern "C"
+
+namespace {
+
+// the reduced matrices and the scratch of a plume whose grid and dimensions are set
+int finish_plume(cfs_plume* P, const double* damping, const double* ptof, const double* inverse_product,
+                 const cfs_plume_params* prm) {
+  if (P->r_pre < 1 || P->r_pre > MAX_R || P->r_fin < 1 || P->r_fin > MAX_R || P->r_adv < 1 || P->r_adv > MAX_R ||
+      P->r_proj < 1 || P->r_proj > MAX_R)
+    return cfs::fail(CFS_ERR_ARG, "every reduced dimension must lie in [1, 256]");
+  if (!(prm->dt > 0.0) || !(prm->dx > 0.0)) return cfs::fail(CFS_ERR_ARG, "dt and dx must be positive");
+  P->prm = *prm;
+  cfs::DeviceGuard g(P->device);
+  const size_t n = (size_t)P->X * P->Y * P->Z;
+  int st;
+  if ((st = dev_alloc(P, &P->d_D, (size_t)P->r_proj * P->r_adv, damping))) return st;
+  if ((st = dev_alloc(P, &P->d_P, (size_t)P->r_fin * P->r_proj, ptof))) return st;
+  if ((st = dev_alloc(P, &P->d_I, (size_t)P->r_fin * P->r_proj, inverse_product))) return st;
+  if ((st = dev_alloc<double>(P, &P->d_q, (size_t)MAX_R, nullptr))) return st;
+  if ((st = dev_alloc<float>(P, &P->d_w, 3 * n, nullptr))) return st;
+  if ((st = dev_alloc<float>(P, &P->d_wmag, n, nullptr))) return st;
+  if ((st = dev_alloc<float>(P, &P->d_dold, n, nullptr))) return st;
+  if ((st = dev_alloc<float>(P, &P->d_hold, n, nullptr))) return st;
+  if ((st = dev_alloc<float>(P, &P->d_fa, 3 * n, nullptr))) return st;
+  if ((st = dev_alloc<float>(P, &P->d_fb, 3 * n, nullptr))) return st;
+  if (P->U_pre_dense && (st = dev_alloc<double>(P, &P->d_col, 3 * (size_t)peeled(P), nullptr))) return st;
+  return CFS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cfs_plume_create(cfs_basis* pre, cfs_basis* fin, int r_adv, int r_proj, const double* damping,
+                     const double* ptof, const double* inverse_product, const cfs_plume_params* prm,
+                     cfs_plume** out) {
   if (!out) return cfs::fail(CFS_ERR_ARG, "null output handle");
   *out = nullptr;
   if (!pre || !fin || !damping || !ptof || !inverse_product || !prm) return cfs::fail(CFS_ERR_ARG, "null argument");
   cfs_basis_info a, b;
   int st = cfs_basis_get_info(pre, &a);
   if (st || (st = cfs_basis_get_info(fin, &b))) return st;
-  if (a.dims[0] != b.dims[0] || a.dims[1] != b.dims[1] || a.dims[2] != b.dims[2] || a.num_cols != b.num_cols)
-    return cfs::fail(CFS_ERR_ARG, "U_preadvect and U_final must share the grid and the column count");
-  if (a.num_cols > MAX_R) return cfs::fail(CFS_ERR_ARG, "more than 256 columns");
+  if (a.dims[0] != b.dims[0] || a.dims[1] != b.dims[1] || a.dims[2] != b.dims[2])
+    return cfs::fail(CFS_ERR_ARG, "U_preadvect and U_final must share the grid");
   if (cfs::basis_device(pre) != cfs::basis_device(fin)) return cfs::fail(CFS_ERR_ARG, "bases on different devices");
-  if (!(prm->dt > 0.0) || !(prm->dx > 0.0)) return cfs::fail(CFS_ERR_ARG, "dt and dx must be positive");
   std::unique_ptr<cfs_plume> P(new cfs_plume);
   P->device = cfs::basis_device(pre);
-  cfs::DeviceGuard g(P->device);
   P->pre = pre;
   P->fin = fin;
-  P->prm = *prm;
   P->X = a.dims[0] + 2;
   P->Y = a.dims[1] + 2;
   P->Z = a.dims[2] + 2;
-  P->r = a.num_cols;
-  const size_t n = (size_t)P->X * P->Y * P->Z, rr = (size_t)P->r * P->r;
-  if ((st = dev_alloc(P.get(), &P->d_D, rr, damping))) return st;
-  if ((st = dev_alloc(P.get(), &P->d_P, rr, ptof))) return st;
-  if ((st = dev_alloc(P.get(), &P->d_I, rr, inverse_product))) return st;
-  if ((st = dev_alloc<double>(P.get(), &P->d_q, (size_t)P->r, nullptr))) return st;
-  if ((st = dev_alloc<float>(P.get(), &P->d_w, 3 * n, nullptr))) return st;
-  if ((st = dev_alloc<float>(P.get(), &P->d_wmag, n, nullptr))) return st;
-  if ((st = dev_alloc<float>(P.get(), &P->d_dold, n, nullptr))) return st;
-  if ((st = dev_alloc<float>(P.get(), &P->d_hold, n, nullptr))) return st;
-  if ((st = dev_alloc<float>(P.get(), &P->d_fa, 3 * n, nullptr))) return st;
-  if ((st = dev_alloc<float>(P.get(), &P->d_fb, 3 * n, nullptr))) return st;
+  P->r_pre = a.num_cols;
+  P->r_adv = r_adv;
+  P->r_proj = r_proj;
+  P->r_fin = b.num_cols;
+  if ((st = finish_plume(P.get(), damping, ptof, inverse_product, prm))) return st;
+  *out = P.release();
+  return CFS_OK;
+}
+
+int cfs_plume_create_dense(const double* U_pre, int r_pre, const double* U_fin, int r_fin, int X, int Y, int Z,
+                           int r_adv, int r_proj, const double* damping, const double* ptof,
+                           const double* inverse_product, const cfs_plume_params* prm, cfs_plume** out) {
+  if (!out) return cfs::fail(CFS_ERR_ARG, "null output handle");
+  *out = nullptr;
+  if (!U_pre || !U_fin || !damping || !ptof || !inverse_product || !prm)
+    return cfs::fail(CFS_ERR_ARG, "null argument");
+  if (X < 2 || Y < 2 || Z < 2) return cfs::fail(CFS_ERR_ARG, "bad peeled grid");
+  std::unique_ptr<cfs_plume> P(new cfs_plume);
+  PL_HIP(hipGetDevice(&P->device), "hipGetDevice");
+  P->U_pre_dense = U_pre;
+  P->U_fin_dense = U_fin;
+  P->X = X + 2;
+  P->Y = Y + 2;
+  P->Z = Z + 2;
+  P->r_pre = r_pre;
+  P->r_adv = r_adv;
+  P->r_proj = r_proj;
+  P->r_fin = r_fin;
+  int st = finish_plume(P.get(), damping, ptof, inverse_product, prm);
+  if (st) return st;
   *out = P.release();
   return CFS_OK;
 }
@@ -183,15 +288,19 @@ int cfs_plume_set_advection(cfs_plume* p, int mode, cfs_cubature* cub, cfs_closu
     if (!cub) return cfs::fail(CFS_ERR_ARG, "cubature mode needs a cubature");
     int st = cfs_cubature_get_info(cub, &K, &r);
     if (st) return st;
-    if (r != p->r) return cfs::fail(CFS_ERR_ARG, "the cubature's after-basis has another column count");
+    if (r != p->r_adv) return cfs::fail(CFS_ERR_ARG, "the cubature's after-basis has another column count");
   } else if (mode == CFS_PLUME_ADVECT_CLOSURE) {
     int r = 0, rp = 0, h = 0;
     int64_t np = 0;
     if (!closure) return cfs::fail(CFS_ERR_ARG, "closure mode needs a closure");
     int st = cfs_closure_get_dims(closure, &r, &rp, &h, &np);
     if (st) return st;
-    if (r != p->r) return cfs::fail(CFS_ERR_ARG, "the closure has another column count");
-  } else if (mode != CFS_PLUME_ADVECT_EXACT) {
+    if (r != p->r_pre || p->r_adv != p->r_pre)
+      return cfs::fail(CFS_ERR_ARG, "the closure maps U_preadvect coordinates to themselves: r must equal both");
+  } else if (mode == CFS_PLUME_ADVECT_EXACT) {
+    if (p->r_adv != p->r_pre)
+      return cfs::fail(CFS_ERR_ARG, "the exact advection stays in U_preadvect coordinates: r_adv must equal its r");
+  } else {
     return cfs::fail(CFS_ERR_ARG, "unknown advection mode");
   }
   p->mode = mode;
@@ -213,6 +322,8 @@ int cfs_plume_add_smoke(cfs_plume* p, float* density, float* heat, void* stream)
 
 int cfs_plume_step(cfs_plume* p, float* velocity, float* density, float* heat, double* q, void* stream) {
   if (!p || !velocity || !density || !heat || !q) return cfs::fail(CFS_ERR_ARG, "null argument");
+  if (p->mode == CFS_PLUME_ADVECT_EXACT && p->r_adv != p->r_pre)
+    return cfs::fail(CFS_ERR_ARG, "no advection set: the exact advection needs r_adv equal to U_preadvect's r");
   cfs::DeviceGuard g(p->device);
   const hipStream_t s = (hipStream_t)stream;
   const int X = p->X, Y = p->Y, Z = p->Z, n = X * Y * Z;
@@ -231,7 +342,7 @@ int cfs_plume_step(cfs_plume* p, float* velocity, float* density, float* heat, d
     PL_HIP(hipGetLastError(), "confinement launch");
   }
   // projection into the subspace
-  if ((st = cfs_project(p->pre, velocity, q, stream))) return st;
+  if ((st = project_pre(p, velocity, q, stream))) return st;
   // density and heat
   hipLaunchKernelGGL(velocity_border_kernel<float>, grid, block, 0, s, velocity, X, Y, Z, p->prm.open_x,
                      p->prm.open_y, p->prm.open_z, 1);
@@ -243,18 +354,30 @@ int cfs_plume_step(cfs_plume* p, float* velocity, float* density, float* heat, d
   hipLaunchKernelGGL(zero_border2_kernel<float>, grid, block, 0, s, density, heat, X, Y, Z);
   PL_HIP(hipGetLastError(), "border launch");
   // advection in the subspace
-  if (p->mode == CFS_PLUME_ADVECT_CUBATURE)
-    st = cfs_cubature_advect(p->cub, p->pre, q, p->d_q, p->d_fa, dt0, 1.0, nullptr, stream);
-  else if (p->mode == CFS_PLUME_ADVECT_CLOSURE)
+  if (p->mode == CFS_PLUME_ADVECT_CLOSURE) {
     st = cfs_closure_step(p->closure, q, p->d_q, 1, 1.0, nullptr, stream);
-  else
-    st = cfs_reduced_advect(p->pre, q, p->d_q, p->d_fa, p->d_fb, dt0, 1.0, nullptr, stream);
+  } else if (!p->U_pre_dense) {
+    st = p->mode == CFS_PLUME_ADVECT_CUBATURE
+             ? cfs_cubature_advect(p->cub, p->pre, q, p->d_q, p->d_fa, dt0, 1.0, nullptr, stream)
+             : cfs_reduced_advect(p->pre, q, p->d_q, p->d_fa, p->d_fb, dt0, 1.0, nullptr, stream);
+  } else {
+    // dense U_preadvect: unproject once (d_fa keeps a zero border), then the
+    // cubature from the field, or the exact advection and the projection
+    if ((st = unproject_with(p, 0, q, p->d_fa, stream))) return st;
+    if (p->mode == CFS_PLUME_ADVECT_CUBATURE) {
+      st = cfs_cubature_advect_field(p->cub, p->d_fa, p->d_q, dt0, 1.0, nullptr, stream);
+    } else {
+      if ((st = cfs_advect_field(p->d_fa, p->d_fb, X, Y, Z, dt0, stream))) return st;
+      st = project_pre(p, p->d_fb, p->d_q, stream);
+    }
+  }
   if (st) return st;
   // diffusion and pressure projection
-  hipLaunchKernelGGL(reduced_ops_kernel, dim3(1), dim3(MAX_R), 0, s, p->d_D, p->d_P, p->d_I, p->d_q, q, p->r);
+  hipLaunchKernelGGL(reduced_ops_kernel, dim3(1), dim3(MAX_R), 0, s, p->d_D, p->d_P, p->d_I, p->d_q, q, p->r_adv,
+                     p->r_proj, p->r_fin);
   PL_HIP(hipGetLastError(), "reduced operators launch");
   // unprojection
-  return cfs_unproject(p->fin, q, velocity, stream);
+  return unproject_with(p, 1, q, velocity, stream);
 }
 
 int cfs_plume_destroy(cfs_plume* p) {

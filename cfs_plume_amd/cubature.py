@@ -59,6 +59,24 @@ class Cubature:
         return cls(h.value)
 
     @classmethod
+    def from_rows(cls, cells, weights, dims, U_after: np.ndarray) -> "Cubature":
+        """With a dense after-basis (U.prediffuse): U_after is r x 3 Npeel (one
+        mode per row, components interleaved) on the host; dims the peeled grid."""
+        c = np.ascontiguousarray(cells, np.int32)
+        w = np.ascontiguousarray(weights, np.float64)
+        if c.shape != w.shape or c.ndim != 1:
+            raise ValueError("cells and weights must be 1-D arrays of one length")
+        U = np.asarray(U_after, np.float64)
+        r = U.shape[0]
+        # entry (comp, j) of cell i at i 3r + comp + 3j
+        rows = np.ascontiguousarray(U.reshape(r, -1, 3)[:, c, :].transpose(1, 0, 2))
+        h = ctypes.c_void_p()
+        X, Y, Z = dims
+        _lib.check(_lib.lib().cfs_cubature_create_rows(len(c), c.ctypes.data, w.ctypes.data, X, Y, Z, r,
+                                                       rows.ctypes.data, ctypes.byref(h)), "readAdvectionCubature")
+        return cls(h.value)
+
+    @classmethod
     def from_file(cls, path: str, after: CompressedBasis) -> "Cubature":
         h = ctypes.c_void_p()
         _lib.check(_lib.lib().cfs_cubature_open_file(path.encode(), after._h, ctypes.byref(h)), "readAdvectionCubature")

@@ -38,21 +38,53 @@ class PlumeSolver:
 
     def __init__(self, pre: CompressedBasis, fin: CompressedBasis, damping: np.ndarray, ptof: np.ndarray,
                  inverse_product: np.ndarray, params: Optional[_lib.PlumeParams] = None):
+        """damping: r_proj x r_adv; ptof, inverse_product: r_fin x r_proj (r_fin = fin.r)."""
         X, Y, Z = pre.dims
         self.full = (X + 2, Y + 2, Z + 2)
         self.r = pre.r
         self.params = params if params is not None else default_params(self.full)
         mats = [np.ascontiguousarray(m, np.float64) for m in (damping, ptof, inverse_product)]
-        for m in mats:
-            if m.shape != (self.r, self.r):
-                raise ValueError(f"reduced matrices must be {self.r} x {self.r}, got {m.shape}")
+        r_proj, r_adv = mats[0].shape
+        self.r_adv, self.r_proj, self.r_fin = r_adv, r_proj, fin.r
+        for m in mats[1:]:
+            if m.shape != (fin.r, r_proj):
+                raise ValueError(f"ptof and inverseProduct must be {fin.r} x {r_proj}, got {m.shape}")
         h = ctypes.c_void_p()
-        _lib.check(_lib.lib().cfs_plume_create(pre._h, fin._h, mats[0].ctypes.data, mats[1].ctypes.data,
-                                               mats[2].ctypes.data, ctypes.byref(self.params), ctypes.byref(h)),
-                   "cfs_plume_create")
+        _lib.check(_lib.lib().cfs_plume_create(pre._h, fin._h, r_adv, r_proj, mats[0].ctypes.data,
+                                               mats[1].ctypes.data, mats[2].ctypes.data, ctypes.byref(self.params),
+                                               ctypes.byref(h)), "cfs_plume_create")
         self._h = h.value
         self.pre, self.fin = pre, fin
         self._keep = ()
+
+    @classmethod
+    def from_dense(cls, U_pre: DeviceArray, U_fin: DeviceArray, peeled_dims, damping: np.ndarray, ptof: np.ndarray,
+                   inverse_product: np.ndarray, params: Optional[_lib.PlumeParams] = None) -> "PlumeSolver":
+        """The uncompressed solver: U_pre, U_fin device FP64 arrays of shape
+        (r, 3 Npeel) (mode j = row j, i.e. 3 Npeel x r column-major)."""
+        self = cls.__new__(cls)
+        X, Y, Z = peeled_dims
+        self.full = (X + 2, Y + 2, Z + 2)
+        self.r = U_pre.shape[0]
+        self.params = params if params is not None else default_params(self.full)
+        mats = [np.ascontiguousarray(m, np.float64) for m in (damping, ptof, inverse_product)]
+        r_proj, r_adv = mats[0].shape
+        self.r_adv, self.r_proj, self.r_fin = r_adv, r_proj, U_fin.shape[0]
+        for m in mats[1:]:
+            if m.shape != (self.r_fin, r_proj):
+                raise ValueError(f"ptof and inverseProduct must be {self.r_fin} x {r_proj}, got {m.shape}")
+        n3 = 3 * X * Y * Z
+        if U_pre.shape[1] != n3 or U_fin.shape[1] != n3:
+            raise ValueError("dense bases must hold 3 Npeel values per mode")
+        h = ctypes.c_void_p()
+        _lib.check(_lib.lib().cfs_plume_create_dense(U_pre.ptr, self.r, U_fin.ptr, self.r_fin, X, Y, Z, r_adv, r_proj,
+                                                     mats[0].ctypes.data, mats[1].ctypes.data, mats[2].ctypes.data,
+                                                     ctypes.byref(self.params), ctypes.byref(h)),
+                   "cfs_plume_create_dense")
+        self._h = h.value
+        self.pre, self.fin = U_pre, U_fin  # kept alive: the plume reads them every step
+        self._keep = ()
+        return self
 
     @classmethod
     def from_operators(cls, pre: CompressedBasis, fin: CompressedBasis, ops, params=None) -> "PlumeSolver":
@@ -68,7 +100,7 @@ class PlumeSolver:
         """Zeroed device state: velocity (3 per cell), density, heat, q."""
         n = self.full[0] * self.full[1] * self.full[2]
         return (DeviceArray.zeros((3 * n,), np.float32), DeviceArray.zeros((n,), np.float32),
-                DeviceArray.zeros((n,), np.float32), DeviceArray.zeros((self.r,), np.float64))
+                DeviceArray.zeros((n,), np.float32), DeviceArray.zeros((max(self.r, self.r_fin),), np.float64))
 
     def add_smoke(self, density: DeviceArray, heat: DeviceArray, stream: Optional[Stream] = None):
         _lib.check(_lib.lib().cfs_plume_add_smoke(self._h, density.ptr, heat.ptr, _sptr(stream)), "addSmokeColumn")

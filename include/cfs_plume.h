@@ -192,6 +192,13 @@ int cfs_cubature_create(int K, const int32_t* cells, const double* weights, cfs_
  * (CUBATURE_GENERATOR_EIGEN::writeCubature, src/cubature/CUBATURE_GENERATOR_EIGEN.cpp:50-91;
  * read by readAdvectionCubature :1195-1232).  A missing file is CFS_ERR_IO,
  * a size that does not match K is CFS_ERR_FORMAT. */
+/* The same with the after-basis dense (the reference reads U.prediffuse.matrix
+ * and takes its rows at the key cells, cellBasisPeeled,
+ * SUBSPACE_FLUID_3D_COMPRESSED_EIGEN.cpp:1280-1292): rows (host) holds, for
+ * key cell i, the 3 x r rows of U_after, entry (c, j) at i 3r + c + 3j;
+ * X, Y, Z the peeled grid. */
+int cfs_cubature_create_rows(int K, const int32_t* cells, const double* weights, int X, int Y, int Z, int r,
+                             const double* rows, cfs_cubature** out);
 int cfs_cubature_open_file(const char* path, cfs_basis* after, cfs_cubature** out);
 int cfs_cubature_write_file(const char* path, int K, const int32_t* cells, const double* weights);
 int cfs_cubature_get_info(const cfs_cubature* c, int* K, int* r);
@@ -201,6 +208,12 @@ int cfs_cubature_get_cells(const cfs_cubature* c, int32_t* cells, double* weight
  * Three launches (unproject, key cells, reduction), no allocation. */
 int cfs_cubature_advect(cfs_cubature* c, cfs_basis* pre, const double* q_in, double* q_out, float* field, float dt0,
                         double alpha, const double* forcing, void* stream);
+/* The same step from the unprojected field U_pre q (FP32 full grid, interior
+ * cells) instead of q: the key-cell interpolation and the sum of the after-
+ * matrices only (accumAdvectRequests + reducedAdvectCompressionFriendly's
+ * sum, :1781-1993), for callers that hold U_pre q already (a dense basis). */
+int cfs_cubature_advect_field(cfs_cubature* c, const float* field, double* q_out, float dt0, double alpha,
+                              const double* forcing, void* stream);
 int cfs_cubature_destroy(cfs_cubature* c);
 
 /* NNLS_SOLVER::solve (src/linearalgebra/NNLS.h:156-208, Lawson-Hanson) as the
@@ -285,10 +298,26 @@ typedef struct {
  * driver's walls (front/back/left/right walls, open top and bottom,
  * reducedCompressed3D.cpp:325-331). */
 void cfs_plume_default_params(int X, int Y, int Z, cfs_plume_params* params);
-/* damping, ptof, inverse_product: host FP64 r x r row-major (the
- * damping.peeled, projected.ptof and inverseProduct matrices). */
-int cfs_plume_create(cfs_basis* pre, cfs_basis* fin, const double* damping, const double* ptof,
+/* The reduced coordinates may differ in size, as the reference's separate
+ * bases do (U_preadvect r_pre columns, the advected coordinates r_adv --
+ * U_prediffuse, the cubature's after-basis --, U_preproject r_proj, U_final
+ * r_fin): damping is r_proj x r_adv (damping.peeled), ptof and
+ * inverse_product are r_fin x r_proj (projected.ptof, inverseProduct), host
+ * FP64 row-major; every dimension <= 256.  The exact advection and the
+ * closure stay in U_preadvect coordinates (r_adv = r_pre).  The q of
+ * cfs_plume_step holds max(r_pre, r_fin) doubles; on return q[0, r_fin) are
+ * the final coordinates. */
+int cfs_plume_create(cfs_basis* pre, cfs_basis* fin, int r_adv, int r_proj, const double* damping, const double* ptof,
                      const double* inverse_product, const cfs_plume_params* params, cfs_plume** out);
+/* The uncompressed solver (SUBSPACE_FLUID_3D_EIGEN::stepPlume, the accuracy
+ * harness's ground truth): the same step with dense bases, device FP64
+ * 3 Npeel x r column-major (one mode per column, components interleaved),
+ * owned by the caller; X, Y, Z the peeled grid.  Project and unproject are
+ * dense products; the advection is the cubature (cfs_cubature_advect_field
+ * on U_pre q), the closure or the exact advection. */
+int cfs_plume_create_dense(const double* U_pre, int r_pre, const double* U_fin, int r_fin, int X, int Y, int Z,
+                           int r_adv, int r_proj, const double* damping, const double* ptof,
+                           const double* inverse_product, const cfs_plume_params* params, cfs_plume** out);
 /* Selects the subspace advection; the cubature or closure must outlive the plume. */
 int cfs_plume_set_advection(cfs_plume* p, int mode, cfs_cubature* cubature, cfs_closure* closure);
 int cfs_plume_add_smoke(cfs_plume* p, float* density, float* heat, void* stream);
@@ -338,6 +367,34 @@ int cfs_fluid_step(cfs_fluid* f, const cfs_fluid_snapshots* snapshots, void* str
 /* CG iterations and residual maximum of the last pressure solve. */
 int cfs_fluid_last_solve(const cfs_fluid* f, int* iterations, double* residual_max);
 int cfs_fluid_destroy(cfs_fluid* f);
+
+/* ---- the basis builder's products (device FP64, column-major) ---------- */
+/* C = A^T B (m x n, row-major, device): A rows x m (leading dimension lda),
+ * B rows x n (ldb).  The snapshot Gram matrices of the SVD
+ * (BIG_MATRIX::outOfCoreSVD, src/linearalgebra/BIG_MATRIX.cpp:276-420) and
+ * the projections U^T (Op V) of buildOutOfCoreMatrices
+ * (src/integrators/SUBSPACE_FLUID_3D_EIGEN.cpp:1355-1489, EIGEN::transposeProduct
+ * src/linearalgebra/EIGEN.cpp:463-472); partial products are reduced in a
+ * fixed order, so the result does not depend on scheduling. */
+int cfs_gemm_tn(const double* A, int64_t rows, int m, int64_t lda, const double* B, int n, int64_t ldb, double* C,
+                void* stream);
+/* C = A M (rows x n, column-major, ldc): A rows x k (lda), M k x n row-major
+ * on the device, k <= 512 (U = Q U_R of BIG_MATRIX.cpp:345-420). */
+int cfs_gemm_nn(const double* A, int64_t rows, int k, int64_t lda, const double* M, int n, double* C, int64_t ldc,
+                void* stream);
+/* The sparse operators of the reduced projection, applied column by column
+ * on the peeled grid of a full grid X x Y x Z (boundaries as in
+ * cfs_fluid_params; dx, and w for the damping):
+ *   CFS_OP_POISSON          buildFlatA (SUBSPACE_FLUID_3D_EIGEN.cpp:2166-2207), Np -> Np
+ *   CFS_OP_VEL_TO_DIV       computeVelocityToDivergence (:1989-2100), 3 Np -> Np
+ *   CFS_OP_PRESSURE_TO_VEL  computePressureToVelocity (:627-695), Np -> 3 Np
+ *   CFS_OP_DAMPING          buildPeeledDampingMatrixFlat (:2107-2160), 3 Np -> 3 Np */
+#define CFS_OP_POISSON 0
+#define CFS_OP_VEL_TO_DIV 1
+#define CFS_OP_PRESSURE_TO_VEL 2
+#define CFS_OP_DAMPING 3
+int cfs_apply_operator(int op, int X, int Y, int Z, const int* boundaries, double dx, double w, const double* in,
+                       int64_t ld_in, int ncols, double* out, int64_t ld_out, void* stream);
 
 /* ---- encoder (host) ---------------------------------------------------------
  * CompressAndWriteMatrixComponents (COMPRESSION.cpp:2271-2359) with the SVD
