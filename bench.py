@@ -38,6 +38,7 @@ DEFAULT = "128x192x128_r200"
 BATCH = "batch16"  # 16 independent bases of seeded sizes in one persistent launch
 CLOSURE = "closure"  # learned advection closure vs the cubature reduced-advection step, 48x64x48 r=52
 PLUME = "plume"  # the whole plume timestep on the device (stepPlume), two bases, HIP-graph replay
+PIPELINE = "pipeline"  # full-rank plume -> SVD bases -> reduced operators -> cubature -> compressed step, accuracy
 ACCURACY = "accuracy"  # compressed vs uncompressed reduced plume, per-step q error, 48x64x48 r=52
 
 
@@ -356,6 +357,84 @@ def run_plume(args):
     print(json.dumps(result), flush=True)
 
 
+def run_pipeline(args):
+    """The reference's whole offline chain on one MI355X (cfg stam.64: a
+    48x64x48 Stam plume, 100 full-rank steps, SVD bases with the 1e-9 keep
+    rule, buildProducts, the staged cubature from 20 snapshot pairs, the
+    bases compressed at percent 0.9995, nBits 32; the cubature trained on all
+    100 snapshot pairs), then the reduced plume
+    stepped 100 frames with the compressed and with the uncompressed bases
+    (compareSubspace per step, the measurement behind the README's
+    0.25 / 0.91 / 1.06 % at steps 1 / 50 / 100), and the fused step timed on
+    the plume's own compressed U_final."""
+    import cfs_plume_amd as C
+    from cfs_plume_amd import accuracy as A
+    from cfs_plume_amd import pipeline as PP
+    from cfs_plume_amd.hip import DeviceArray, Stream
+
+    grid = tuple(int(g) for g in args.pipeline_grid.split("x"))
+    steps = args.pipeline_steps
+    t0 = time.perf_counter()
+    data = PP.build_plume_data(grid, steps, cubature_snapshots=args.pipeline_cubature_snapshots,
+                               candidates=args.pipeline_candidates, tolerance=args.pipeline_tolerance,
+                               threads=args.threads, log=lambda m: log(f"[bench] {m}"))
+    t_build = time.perf_counter() - t0
+    ranks = {k: b.r for k, b in data.bases.items()}
+    log(f"[bench] pipeline built in {t_build:.1f}s: ranks {ranks}, cubature {len(data.cubature.cells)} cells "
+        f"(error {data.cubature.relative_error:.3e}), timings {data.timings}")
+    if args.pipeline_out:
+        PP.write_reduced(data, args.pipeline_out)
+    t0 = time.perf_counter()
+    acc = A.compare_plume(data, steps, out_prefix=os.path.join(args.pipeline_out, "run") if args.pipeline_out else None)
+    t_runs = time.perf_counter() - t0
+    e = acc["q_rel_l2"]
+    pick = {str(s): round(float(e[s - 1]), 6) for s in (1, 10, 50, 100) if s <= steps}
+    vc, vu = acc["compressed"]["velocity_vs_full_rank"], acc["uncompressed"]["velocity_vs_full_rank"]
+    for name in ("compressed", "uncompressed", "uncompressed_exact"):
+        if name in acc:
+            v = acc[name]["velocity_vs_full_rank"]
+            log(f"[bench] {name} velocity vs full rank: " +
+                " ".join(f"{s}:{v[s - 1]:.3e}" for s in (1, 2, 5, 10, 20, 50, 100) if s <= len(v)))
+    log(f"[bench] compressed vs uncompressed q: {pick} (max {e.max():.3e}); velocity vs full rank at the last "
+        f"step: uncompressed {vu[-1]:.3e}, compressed {vc[-1]:.3e} ({t_runs:.1f}s)")
+    # the fused step on the plume's own compressed U_final
+    U = C.CompressedBasis.from_buffers(data.compressed["final"])
+    r = U.r
+    stream = Stream()
+    rng = np.random.default_rng(42)
+    n3 = 3 * (grid[0] * grid[1] * grid[2])
+    dv = DeviceArray.from_numpy(rng.standard_normal(n3).astype(np.float32))
+    dq = DeviceArray.from_numpy(rng.standard_normal(r))
+    qo = DeviceArray.zeros((r,), np.float64)
+    fo = DeviceArray.zeros((n3,), np.float32)
+    ms, per = time_launches(lambda: C.step(U, U, dv, qo, dq, fo, stream), args.steps, args.warmup, stream)
+    U.raise_on_stream_errors()
+    step_bytes = U.step_bytes()
+    result = {
+        "metric": f"compressed vs uncompressed reduced q, relative L2 at step {steps} (stam plume "
+                  f"{grid[0]}x{grid[1]}x{grid[2]}, percent 0.9995)",
+        "value": round(float(e[-1]), 6), "unit": "relative L2", "n_gpus": 1, "steps": steps, "warmup": 0,
+        "ms_per_step": None, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp64 full-rank and bases, fp32 reduced fields, fp64 q", "data": "simulated (full-rank plume on the device)",
+        "config": {
+            "workload": f"fluid3D -> svdOutOfCoreMultiple -> buildProducts -> cubatureGeneratorStamStaged -> "
+                        f"compressMatrices -> reducedCompressed3D vs reduced3D, {steps} frames",
+            "grid": list(grid), "ranks": ranks, "parallelism": "single GPU",
+        },
+        "accuracy": {"q_rel_l2": pick, "q_rel_l2_max": round(float(e.max()), 6),
+                     "reference_readme_rel_l2": {"1": 0.0025, "50": 0.0091, "100": 0.0106},
+                     "velocity_vs_full_rank_last": {"uncompressed": round(float(vu[-1]), 6),
+                                                    "compressed": round(float(vc[-1]), 6)}},
+        "cubature": {"key_cells": int(len(data.cubature.cells)), "training_rel_error": data.cubature.relative_error,
+                     "passes": data.cubature.passes},
+        "fused_step_on_plume_basis": {"ms": round(ms, 5), "median_ms": round(float(np.median(per)), 5),
+                                      "r": r, "bytes": step_bytes,
+                                      "GB_per_s": round(step_bytes / (ms * 1e-3) / 1e9, 1)},
+        "timings_s": {k: round(v, 2) for k, v in data.timings.items()} | {"reduced_runs": round(t_runs, 2)},
+    }
+    print(json.dumps(result), flush=True)
+
+
 def run_accuracy(args):
     """Compression error of the reduced plume (the measurement of the
     reference's reducedCompressed3DCompareVelocities): the 48x64x48, r = 52
@@ -458,7 +537,15 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default=DEFAULT, choices=sorted(WORKLOADS) + [BATCH, CLOSURE, ACCURACY, PLUME])
+    ap.add_argument("--workload", default=DEFAULT, choices=sorted(WORKLOADS) + [BATCH, CLOSURE, ACCURACY, PLUME, PIPELINE])
+    ap.add_argument("--pipeline-grid", default="48x64x48", help="full grid of the pipeline workload")
+    ap.add_argument("--pipeline-steps", type=int, default=100, help="full-rank steps and reduced frames")
+    ap.add_argument("--pipeline-cubature-snapshots", type=int, default=100,
+                    help="snapshot pairs the cubature is trained on (cfg 'reduced snapshots'; the default 20 "
+                         "of the reference's parser trains on the early plume only, and the reduced run diverges)")
+    ap.add_argument("--pipeline-candidates", type=int, default=1000, help="cubature candidates per pass")
+    ap.add_argument("--pipeline-tolerance", type=float, default=0.01, help="cubature error that ends the passes")
+    ap.add_argument("--pipeline-out", default="", help="write the reduced files and q trajectories here")
     ap.add_argument("--plume-grid", default=DEFAULT, choices=sorted(WORKLOADS),
                     help="grid and r of the plume workload")
     ap.add_argument("--accuracy-runs", type=int, default=4, help="seeded runs of the accuracy workload")
@@ -502,6 +589,9 @@ def main():
         return
     if args.workload == PLUME:
         run_plume(args)
+        return
+    if args.workload == PIPELINE:
+        run_pipeline(args)
         return
 
     import cfs_plume_amd as C

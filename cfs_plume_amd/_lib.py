@@ -74,6 +74,7 @@ SIGNATURES = {
     "cfs_plume_default_params": (None, [_i, _i, _i, _vp]),
     "cfs_plume_create": (_i, [_vp, _vp, _i, _i, _vp, _vp, _vp, _vp, ctypes.POINTER(_vp)]),
     "cfs_plume_create_dense": (_i, [_vp, _i, _vp, _i, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, ctypes.POINTER(_vp)]),
+    "cfs_plume_set_exact_after": (_i, [_vp, _vp]),
     "cfs_plume_set_advection": (_i, [_vp, _i, _vp, _vp]),
     "cfs_plume_add_smoke": (_i, [_vp, _vp, _vp, _vp]),
     "cfs_plume_step": (_i, [_vp, _vp, _vp, _vp, _vp, _vp]),

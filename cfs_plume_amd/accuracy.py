@@ -24,7 +24,7 @@ from .closure import PlumeRun, ReducedPlume, advect_field
 from .hip import Stream
 from .matrix_io import TrajectoryWriter
 
-__all__ = ["relative_l2_per_step", "DensePlume", "compare_runs"]
+__all__ = ["relative_l2_per_step", "DensePlume", "compare_runs", "compare_plume"]
 
 
 def relative_l2_per_step(q_true: np.ndarray, q: np.ndarray) -> np.ndarray:
@@ -102,3 +102,66 @@ def compare_runs(plume: ReducedPlume, U_dense: np.ndarray, runs, steps: int, out
                 for t in range(1, steps + 1):
                     w.append(traj[t])
     return errs
+
+
+def compare_plume(data, steps: int, stream: Optional[Stream] = None, out_prefix: Optional[str] = None,
+                  exact: bool = True) -> dict:
+    """The reference's measurement on a real plume (reducedCompressed3DCompareVelocities.cpp:
+    379-433): the whole reduced step (`stepPlume`, addSmokeColumn every frame,
+    the cubature advection) run with the compressed U_preadvect / U_final on
+    the device and with the dense bases they were encoded from (the
+    uncompressed solver, dense products on the device), from rest;
+    compareSubspace per step, and the reduced velocity of each run against
+    the full-rank run's final velocity of the same step (diffGroundTruth).
+
+    `data` is a pipeline.PlumeData.  Returns per-step arrays."""
+    from .cubature import Cubature
+    from .hip import DeviceArray
+    from .pipeline import compressed_bases
+    from .plume import ADVECT_CUBATURE, ADVECT_EXACT, PlumeSolver, default_params
+
+    dims = data.peeled
+    ops = data.operators
+    D, P, I = ops["damping.peeled"], ops["projected.ptof"], ops["inverseProduct"]
+    prm = default_params(data.full)
+    prm.vorticity_eps = 0.0  # the drivers' cfg value ("vorticity", default 0)
+    cub = Cubature.from_rows(data.cubature.cells, data.cubature.weights, dims, data.bases["prediffuse"].numpy())
+    comp = compressed_bases(data)
+    runs = {
+        "compressed": PlumeSolver(comp["preadvect"], comp["final"], D, P, I, prm),
+        "uncompressed": PlumeSolver.from_dense(data.bases["preadvect"].U, data.bases["final"].U, dims, D, P, I, prm),
+    }
+    if exact:
+        # the uncompressed solver with the exact reduced advection U_prediffuse^T A(U_preadvect q):
+        # what the cubature approximates
+        runs["uncompressed_exact"] = PlumeSolver.from_dense(data.bases["preadvect"].U, data.bases["final"].U, dims,
+                                                            D, P, I, prm)
+        runs["uncompressed_exact"].set_exact_after(data.bases["prediffuse"].U)
+    truth = data.snapshots.arrays["final_velocity"].numpy()
+    r_fin = data.bases["final"].r
+    out = {}
+    for name, solver in runs.items():
+        if name == "uncompressed_exact":
+            solver.set_advection(ADVECT_EXACT)
+        else:
+            solver.set_advection(ADVECT_CUBATURE, cubature=cub)
+        vel, dens, heat, q = solver.fields()
+        qs = np.zeros((steps, r_fin))
+        vdiff = np.zeros(steps)
+        X, Y, Z = data.full
+        for t in range(steps):
+            solver.frame(vel, dens, heat, q, stream)
+            qs[t] = q.numpy()[:r_fin]
+            v = vel.numpy().reshape(Z, Y, X, 3)[1:-1, 1:-1, 1:-1].reshape(-1).astype(np.float64)
+            if t < truth.shape[0]:
+                vdiff[t] = np.linalg.norm(v - truth[t]) / max(np.linalg.norm(truth[t]), 1e-300)
+        out[name] = {"q": qs, "velocity_vs_full_rank": vdiff}
+        solver.close()
+    if out_prefix:
+        for name in runs:
+            w = TrajectoryWriter(f"{out_prefix}.{name}.qDot.matrix", steps, r_fin)
+            for row in out[name]["q"]:
+                w.append(row)
+    out["q_rel_l2"] = relative_l2_per_step(out["uncompressed"]["q"], out["compressed"]["q"])
+    cub.close()
+    return out

@@ -120,6 +120,7 @@ struct cfs_plume {
   // owned by the caller; null for compressed bases
   const double* U_pre_dense = nullptr;
   const double* U_fin_dense = nullptr;
+  const double* U_adv_dense = nullptr;  // dense exact advection: project onto this after-basis
   double* d_col = nullptr;  // an FP64 peeled vector field
   float *d_w = nullptr, *d_wmag = nullptr, *d_dold = nullptr, *d_hold = nullptr, *d_fa = nullptr, *d_fb = nullptr;
   std::vector<void*> allocs;
@@ -281,6 +282,13 @@ int cfs_plume_create_dense(const double* U_pre, int r_pre, const double* U_fin, 
   return CFS_OK;
 }
 
+int cfs_plume_set_exact_after(cfs_plume* p, const double* U_adv) {
+  if (!p || !U_adv) return cfs::fail(CFS_ERR_ARG, "null argument");
+  if (!p->U_pre_dense) return cfs::fail(CFS_ERR_ARG, "the exact after-basis needs a dense plume");
+  p->U_adv_dense = U_adv;
+  return CFS_OK;
+}
+
 int cfs_plume_set_advection(cfs_plume* p, int mode, cfs_cubature* cub, cfs_closure* closure) {
   if (!p) return cfs::fail(CFS_ERR_ARG, "null plume");
   if (mode == CFS_PLUME_ADVECT_CUBATURE) {
@@ -298,7 +306,7 @@ int cfs_plume_set_advection(cfs_plume* p, int mode, cfs_cubature* cub, cfs_closu
     if (r != p->r_pre || p->r_adv != p->r_pre)
       return cfs::fail(CFS_ERR_ARG, "the closure maps U_preadvect coordinates to themselves: r must equal both");
   } else if (mode == CFS_PLUME_ADVECT_EXACT) {
-    if (p->r_adv != p->r_pre)
+    if (p->r_adv != p->r_pre && !p->U_adv_dense)
       return cfs::fail(CFS_ERR_ARG, "the exact advection stays in U_preadvect coordinates: r_adv must equal its r");
   } else {
     return cfs::fail(CFS_ERR_ARG, "unknown advection mode");
@@ -322,7 +330,7 @@ int cfs_plume_add_smoke(cfs_plume* p, float* density, float* heat, void* stream)
 
 int cfs_plume_step(cfs_plume* p, float* velocity, float* density, float* heat, double* q, void* stream) {
   if (!p || !velocity || !density || !heat || !q) return cfs::fail(CFS_ERR_ARG, "null argument");
-  if (p->mode == CFS_PLUME_ADVECT_EXACT && p->r_adv != p->r_pre)
+  if (p->mode == CFS_PLUME_ADVECT_EXACT && p->r_adv != p->r_pre && !p->U_adv_dense)
     return cfs::fail(CFS_ERR_ARG, "no advection set: the exact advection needs r_adv equal to U_preadvect's r");
   cfs::DeviceGuard g(p->device);
   const hipStream_t s = (hipStream_t)stream;
@@ -368,7 +376,15 @@ int cfs_plume_step(cfs_plume* p, float* velocity, float* density, float* heat, d
       st = cfs_cubature_advect_field(p->cub, p->d_fa, p->d_q, dt0, 1.0, nullptr, stream);
     } else {
       if ((st = cfs_advect_field(p->d_fa, p->d_fb, X, Y, Z, dt0, stream))) return st;
-      st = project_pre(p, p->d_fb, p->d_q, stream);
+      if (p->U_adv_dense) {
+        const long np = peeled(p);
+        hipLaunchKernelGGL(peel_to_column_kernel, dim3((unsigned)((np + TPB - 1) / TPB)), dim3(TPB), 0, s, p->d_fb,
+                           p->d_col, X, Y, Z);
+        PL_HIP(hipGetLastError(), "peel launch");
+        st = cfs_gemm_tn(p->U_adv_dense, 3 * np, p->r_adv, 3 * np, p->d_col, 1, 3 * np, p->d_q, stream);
+      } else {
+        st = project_pre(p, p->d_fb, p->d_q, stream);
+      }
     }
   }
   if (st) return st;

@@ -96,6 +96,11 @@ class PlumeSolver:
                                                       closure.handle if closure else None), "cfs_plume_set_advection")
         self._keep = (cubature, closure)
 
+    def set_exact_after(self, U_adv: DeviceArray):
+        """Dense plumes: the exact advection projects onto U_adv (r_adv, 3 Npeel)."""
+        _lib.check(_lib.lib().cfs_plume_set_exact_after(self._h, U_adv.ptr), "cfs_plume_set_exact_after")
+        self._after = U_adv
+
     def fields(self):
         """Zeroed device state: velocity (3 per cell), density, heat, q."""
         n = self.full[0] * self.full[1] * self.full[2]

@@ -318,6 +318,11 @@ int cfs_plume_create(cfs_basis* pre, cfs_basis* fin, int r_adv, int r_proj, cons
 int cfs_plume_create_dense(const double* U_pre, int r_pre, const double* U_fin, int r_fin, int X, int Y, int Z,
                            int r_adv, int r_proj, const double* damping, const double* ptof,
                            const double* inverse_product, const cfs_plume_params* params, cfs_plume** out);
+/* Dense plumes only: the exact advection projects the advected field onto
+ * this dense after-basis (device FP64, 3 Npeel x r_adv column-major, e.g.
+ * U_prediffuse) instead of U_preadvect, q_adv = U_adv^T A(U_pre q) -- the
+ * quantity the cubature approximates. */
+int cfs_plume_set_exact_after(cfs_plume* p, const double* U_adv);
 /* Selects the subspace advection; the cubature or closure must outlive the plume. */
 int cfs_plume_set_advection(cfs_plume* p, int mode, cfs_cubature* cubature, cfs_closure* closure);
 int cfs_plume_add_smoke(cfs_plume* p, float* density, float* heat, void* stream);
