@@ -39,6 +39,7 @@ BATCH = "batch16"  # 16 independent bases of seeded sizes in one persistent laun
 CLOSURE = "closure"  # learned advection closure vs the cubature reduced-advection step, 48x64x48 r=52
 PLUME = "plume"  # the whole plume timestep on the device (stepPlume), two bases, HIP-graph replay
 PIPELINE = "pipeline"  # full-rank plume -> SVD bases -> reduced operators -> cubature -> compressed step, accuracy
+PLUME_CLOSURE = "plume-closure"  # the learned closure inside the real reduced plume, against its cubature
 ACCURACY = "accuracy"  # compressed vs uncompressed reduced plume, per-step q error, 48x64x48 r=52
 
 
@@ -435,6 +436,64 @@ def run_pipeline(args):
     print(json.dumps(result), flush=True)
 
 
+def run_plume_closure(args):
+    """C5 on the plume's own bases: the pipeline's 48x64x48 plume (compressed
+    U_preadvect r = 52, U_final r = 46, the staged cubature into U_prediffuse
+    coordinates r = 53); 200 seeded reduced-plume runs of 100 frames (source
+    offsets uniform in +-8 cells, buoyancy 0.5-2x) with the cubature train
+    the MLP (r = 53, 3 x 256, GELU) on the device; 20 held-out runs with the
+    closure inside stepPlume are compared with their cubature runs, and the
+    advection stage is timed both ways."""
+    from cfs_plume_amd import pipeline as PP
+    from cfs_plume_amd import plume_closure as PC
+
+    grid = tuple(int(g) for g in args.pipeline_grid.split("x"))
+    data = PP.build_plume_data(grid, args.pipeline_steps, cubature_snapshots=args.pipeline_cubature_snapshots,
+                               candidates=args.pipeline_candidates, tolerance=args.pipeline_tolerance,
+                               threads=args.threads, log=lambda m: log(f"[bench] {m}"))
+    ex = PC.PlumeClosureExperiment(data, frames=100)
+    runs = PC.seeded_plume_runs(220, seed=2026)
+    cl, info = ex.train(runs[:200], epochs=args.plume_closure_epochs, noise=args.plume_closure_noise,
+                        log=lambda m: log(f"[bench] {m}"))
+    log(f"[bench] plume closure: {info['samples']} samples in {info['data_s']:.1f}s, trained in "
+        f"{info['train_s']:.1f}s, mse {info['losses'][0]:.3e} -> {info['losses'][-1]:.3e}")
+    ev = ex.evaluate(cl, runs[200:])
+    d = ev["drift"]
+    times = ex.step_times(cl, args.steps)
+    c, o = ev["checked_drift"], ev["one_step"]
+    log(f"[bench] checked every 10 frames: step 100 mean {c[:, -1].mean():.3e} (max {c[:, -1].max():.3e}), "
+        f"{ev['fallback_frames']} fallback frames of {d.size}; one-step error mean {o.mean():.3e} "
+        f"(median {np.median(o):.3e}, max {o.max():.3e})")
+    log(f"[bench] held-out drift against the cubature runs: step 100 mean {d[:, -1].mean():.3e} (max "
+        f"{d[:, -1].max():.3e}), worst frame mean {d.mean(axis=0).max():.3e}; advection stage: closure "
+        f"{times['closure_ms'] * 1e3:.1f} us, cubature {times['cubature_ms'] * 1e3:.1f} us "
+        f"({times['cubature_ms'] / times['closure_ms']:.1f}x)")
+    result = {
+        "metric": "learned closure inside the reduced plume: advection-stage speedup over the cubature "
+                  "(r_pre=52 -> r_adv=53 @48x64x48 plume basis)",
+        "value": round(times["cubature_ms"] / times["closure_ms"], 2), "unit": "x", "n_gpus": 1,
+        "steps": args.steps, "warmup": 1, "ms_per_step": round(times["closure_ms"], 5), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "bf16 weights, fp32 accumulation",
+        "data": "simulated (full-rank plume -> reduced plume runs on the device)",
+        "config": {"workload": "C5 on the plume's own bases: 200 training runs x 100 frames, 20 held out",
+                   "grid": list(grid), "parallelism": "single GPU"},
+        "closure": {"drift_at_100_mean": round(float(d[:, -1].mean()), 6),
+                    "drift_at_100_max": round(float(d[:, -1].max()), 6),
+                    "drift_worst_frame_mean": round(float(d.mean(axis=0).max()), 6),
+                    "checked_every_10": {"drift_at_100_mean": round(float(c[:, -1].mean()), 6),
+                                         "fallback_frames": int(ev["fallback_frames"]), "frames": int(d.size)},
+                    "one_step_error": {"mean": round(float(o.mean()), 6), "median": round(float(np.median(o)), 6),
+                                       "max": round(float(o.max()), 6)},
+                    "closure_ms": round(times["closure_ms"], 5), "cubature_ms": round(times["cubature_ms"], 5),
+                    "samples": info["samples"], "epochs": args.plume_closure_epochs,
+                    "perturbed_states": args.plume_closure_noise,
+                    "mse_first": info["losses"][0], "mse_last": info["losses"][-1],
+                    "data_s": round(info["data_s"], 2), "train_s": round(info["train_s"], 2)},
+    }
+    ex.close()
+    print(json.dumps(result), flush=True)
+
+
 def run_accuracy(args):
     """Compression error of the reduced plume (the measurement of the
     reference's reducedCompressed3DCompareVelocities): the 48x64x48, r = 52
@@ -537,7 +596,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default=DEFAULT, choices=sorted(WORKLOADS) + [BATCH, CLOSURE, ACCURACY, PLUME, PIPELINE])
+    ap.add_argument("--workload", default=DEFAULT, choices=sorted(WORKLOADS) + [BATCH, CLOSURE, ACCURACY, PLUME, PIPELINE, PLUME_CLOSURE])
     ap.add_argument("--pipeline-grid", default="48x64x48", help="full grid of the pipeline workload")
     ap.add_argument("--pipeline-steps", type=int, default=100, help="full-rank steps and reduced frames")
     ap.add_argument("--pipeline-cubature-snapshots", type=int, default=100,
@@ -546,6 +605,9 @@ def main():
     ap.add_argument("--pipeline-candidates", type=int, default=1000, help="cubature candidates per pass")
     ap.add_argument("--pipeline-tolerance", type=float, default=0.01, help="cubature error that ends the passes")
     ap.add_argument("--pipeline-out", default="", help="write the reduced files and q trajectories here")
+    ap.add_argument("--plume-closure-epochs", type=int, default=3000)
+    ap.add_argument("--plume-closure-noise", type=float, default=0.02,
+                    help="relative size of the perturbed states of the plume-closure workload")
     ap.add_argument("--plume-grid", default=DEFAULT, choices=sorted(WORKLOADS),
                     help="grid and r of the plume workload")
     ap.add_argument("--accuracy-runs", type=int, default=4, help="seeded runs of the accuracy workload")
@@ -592,6 +654,9 @@ def main():
         return
     if args.workload == PIPELINE:
         run_pipeline(args)
+        return
+    if args.workload == PLUME_CLOSURE:
+        run_plume_closure(args)
         return
 
     import cfs_plume_amd as C

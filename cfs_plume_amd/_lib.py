@@ -78,6 +78,8 @@ SIGNATURES = {
     "cfs_plume_set_advection": (_i, [_vp, _i, _vp, _vp]),
     "cfs_plume_add_smoke": (_i, [_vp, _vp, _vp, _vp]),
     "cfs_plume_step": (_i, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "cfs_plume_set_params": (_i, [_vp, _vp]),
+    "cfs_plume_advection_state": (_i, [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp)]),
     "cfs_plume_destroy": (_i, [_vp]),
     "cfs_fluid_default_params": (None, [_i, _i, _i, _vp]),
     "cfs_fluid_create": (_i, [_i, _i, _i, _vp, ctypes.POINTER(_vp)]),
@@ -109,6 +111,8 @@ class PlumeParams(ctypes.Structure):
         ("open_x", ctypes.c_int),
         ("open_y", ctypes.c_int),
         ("open_z", ctypes.c_int),
+        ("smoke_offset_x", ctypes.c_double),
+        ("smoke_offset_z", ctypes.c_double),
     ]
 
 

@@ -541,7 +541,7 @@ int cfs_fluid_add_smoke(cfs_fluid* f, void* stream) {
   cfs::DeviceGuard g(f->device);
   const int y0 = (int)(0.05 * f->Y), y1 = (int)(0.10 * f->Y);
   hipLaunchKernelGGL(smoke_kernel<double>, dim3(grid_blocks(f->n)), dim3(TPB), 0, (hipStream_t)stream, f->dens,
-                     f->heat, f->X, f->Y, f->Z, y0, y1);
+                     f->heat, f->X, f->Y, f->Z, y0, y1, 0.0, 0.0);
   FL_HIP(hipGetLastError(), "smoke launch");
   return CFS_OK;
 }

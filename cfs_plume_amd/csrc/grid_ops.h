@@ -169,9 +169,11 @@ __global__ void __launch_bounds__(GRID_TPB) zero_border2_kernel(T* __restrict__ 
 // addSmokeTestCase (FLUID_3D_MIC.cpp:674-703) on density and heat: 1 inside
 // the column of radius 0.075 xTotal around (0.4 xTotal, 0.5 zTotal) for
 // y in [(int)(0.05 Y), (int)(0.10 Y)], lengths in units of dx = 1 / maxRes.
+// (ox, oz): a shift of the column's axis in cells (0 for the reference's
+// source; the seeded training runs of the learned closure move it).
 template <typename T>
 __global__ void __launch_bounds__(GRID_TPB) smoke_kernel(T* __restrict__ dens, T* __restrict__ heat, int X, int Y,
-                                                         int Z, int y0, int y1) {
+                                                         int Z, int y0, int y1, double ox, double oz) {
   const int i = blockIdx.x * GRID_TPB + threadIdx.x;
   if (i >= X * Y * Z) return;
   const int x = i % X, y = (i / X) % Y, z = i / (X * Y);
@@ -179,8 +181,8 @@ __global__ void __launch_bounds__(GRID_TPB) smoke_kernel(T* __restrict__ dens, T
   const int maxres = max(X, max(Y, Z));
   const double dx = 1.0 / (double)(float)maxres;
   const double xtot = dx * X, ztot = dx * Z;
-  const double xl = x * dx - xtot * (double)0.4f;
-  const double zl = z * dx - ztot * (double)0.5f;
+  const double xl = x * dx - xtot * (double)0.4f - ox * dx;
+  const double zl = z * dx - ztot * (double)0.5f - oz * dx;
   const float radius = sqrtf((float)(xl * xl + zl * zl));
   if ((double)radius < (double)0.075f * xtot) {
     dens[i] = T(1);

@@ -77,6 +77,10 @@ __global__ void __launch_bounds__(TPB) column_to_interior_kernel(const double* _
   for (int c = 0; c < 3; c++) f[3 * i + c] = (float)col[3 * (size_t)j + c];
 }
 
+__global__ void copy_q_kernel(const double* __restrict__ a, double* __restrict__ b, int r) {
+  if ((int)threadIdx.x < r) b[threadIdx.x] = a[threadIdx.x];
+}
+
 // q <- D q (r_proj x r_adv), then q <- P q + I q (r_fin x r_proj), FP64,
 // every dimension <= 256, one workgroup.  q_in and q_out may alias.
 __global__ void __launch_bounds__(MAX_R) reduced_ops_kernel(const double* __restrict__ D, const double* __restrict__ P,
@@ -116,6 +120,7 @@ struct cfs_plume {
   cfs_cubature* cub = nullptr;
   cfs_closure* closure = nullptr;
   double *d_D = nullptr, *d_P = nullptr, *d_I = nullptr, *d_q = nullptr;
+  double* d_qpre = nullptr;  // the advection's input: q after the projection, zero-padded to MAX_R
   // dense bases (the uncompressed solver): device FP64, 3 Npeel x r column-major,
   // owned by the caller; null for compressed bases
   const double* U_pre_dense = nullptr;
@@ -191,6 +196,8 @@ void cfs_plume_default_params(int X, int Y, int Z, cfs_plume_params* prm) {
   prm->buoyancy = 0.1;
   const double scaling = std::max(1.0, (double)(64.0f / (float)maxres));
   prm->vorticity_eps = 2.0 / scaling;
+  prm->smoke_offset_x = 0.0;
+  prm->smoke_offset_z = 0.0;
   prm->open_x = 0;
   prm->open_y = 1;
   prm->open_z = 0;
@@ -215,6 +222,7 @@ int finish_plume(cfs_plume* P, const double* damping, const double* ptof, const 
   if ((st = dev_alloc(P, &P->d_P, (size_t)P->r_fin * P->r_proj, ptof))) return st;
   if ((st = dev_alloc(P, &P->d_I, (size_t)P->r_fin * P->r_proj, inverse_product))) return st;
   if ((st = dev_alloc<double>(P, &P->d_q, (size_t)MAX_R, nullptr))) return st;
+  if ((st = dev_alloc<double>(P, &P->d_qpre, (size_t)MAX_R, nullptr))) return st;
   if ((st = dev_alloc<float>(P, &P->d_w, 3 * n, nullptr))) return st;
   if ((st = dev_alloc<float>(P, &P->d_wmag, n, nullptr))) return st;
   if ((st = dev_alloc<float>(P, &P->d_dold, n, nullptr))) return st;
@@ -303,8 +311,8 @@ int cfs_plume_set_advection(cfs_plume* p, int mode, cfs_cubature* cub, cfs_closu
     if (!closure) return cfs::fail(CFS_ERR_ARG, "closure mode needs a closure");
     int st = cfs_closure_get_dims(closure, &r, &rp, &h, &np);
     if (st) return st;
-    if (r != p->r_pre || p->r_adv != p->r_pre)
-      return cfs::fail(CFS_ERR_ARG, "the closure maps U_preadvect coordinates to themselves: r must equal both");
+    if (r != p->r_adv || r < p->r_pre)
+      return cfs::fail(CFS_ERR_ARG, "the closure's r must equal r_adv and be at least U_preadvect's r");
   } else if (mode == CFS_PLUME_ADVECT_EXACT) {
     if (p->r_adv != p->r_pre && !p->U_adv_dense)
       return cfs::fail(CFS_ERR_ARG, "the exact advection stays in U_preadvect coordinates: r_adv must equal its r");
@@ -322,8 +330,8 @@ int cfs_plume_add_smoke(cfs_plume* p, float* density, float* heat, void* stream)
   cfs::DeviceGuard g(p->device);
   const int n = p->X * p->Y * p->Z;
   const int y0 = (int)(0.05 * p->Y), y1 = (int)(0.10 * p->Y);
-  hipLaunchKernelGGL(smoke_kernel<float>, dim3((n + TPB - 1) / TPB), dim3(TPB), 0, (hipStream_t)stream, density, heat, p->X,
-                     p->Y, p->Z, y0, y1);
+  hipLaunchKernelGGL(smoke_kernel<float>, dim3((n + TPB - 1) / TPB), dim3(TPB), 0, (hipStream_t)stream, density, heat,
+                     p->X, p->Y, p->Z, y0, y1, p->prm.smoke_offset_x, p->prm.smoke_offset_z);
   PL_HIP(hipGetLastError(), "smoke launch");
   return CFS_OK;
 }
@@ -349,8 +357,10 @@ int cfs_plume_step(cfs_plume* p, float* velocity, float* density, float* heat, d
                        (float)(p->prm.vorticity_eps * p->prm.dx));
     PL_HIP(hipGetLastError(), "confinement launch");
   }
-  // projection into the subspace
+  // projection into the subspace; the advection reads its copy, zero-padded
   if ((st = project_pre(p, velocity, q, stream))) return st;
+  hipLaunchKernelGGL(copy_q_kernel, dim3(1), dim3(MAX_R), 0, s, q, p->d_qpre, p->r_pre);
+  PL_HIP(hipGetLastError(), "copy launch");
   // density and heat
   hipLaunchKernelGGL(velocity_border_kernel<float>, grid, block, 0, s, velocity, X, Y, Z, p->prm.open_x,
                      p->prm.open_y, p->prm.open_z, 1);
@@ -363,15 +373,15 @@ int cfs_plume_step(cfs_plume* p, float* velocity, float* density, float* heat, d
   PL_HIP(hipGetLastError(), "border launch");
   // advection in the subspace
   if (p->mode == CFS_PLUME_ADVECT_CLOSURE) {
-    st = cfs_closure_step(p->closure, q, p->d_q, 1, 1.0, nullptr, stream);
+    st = cfs_closure_step(p->closure, p->d_qpre, p->d_q, 1, 1.0, nullptr, stream);
   } else if (!p->U_pre_dense) {
     st = p->mode == CFS_PLUME_ADVECT_CUBATURE
-             ? cfs_cubature_advect(p->cub, p->pre, q, p->d_q, p->d_fa, dt0, 1.0, nullptr, stream)
-             : cfs_reduced_advect(p->pre, q, p->d_q, p->d_fa, p->d_fb, dt0, 1.0, nullptr, stream);
+             ? cfs_cubature_advect(p->cub, p->pre, p->d_qpre, p->d_q, p->d_fa, dt0, 1.0, nullptr, stream)
+             : cfs_reduced_advect(p->pre, p->d_qpre, p->d_q, p->d_fa, p->d_fb, dt0, 1.0, nullptr, stream);
   } else {
     // dense U_preadvect: unproject once (d_fa keeps a zero border), then the
     // cubature from the field, or the exact advection and the projection
-    if ((st = unproject_with(p, 0, q, p->d_fa, stream))) return st;
+    if ((st = unproject_with(p, 0, p->d_qpre, p->d_fa, stream))) return st;
     if (p->mode == CFS_PLUME_ADVECT_CUBATURE) {
       st = cfs_cubature_advect_field(p->cub, p->d_fa, p->d_q, dt0, 1.0, nullptr, stream);
     } else {
@@ -394,6 +404,20 @@ int cfs_plume_step(cfs_plume* p, float* velocity, float* density, float* heat, d
   PL_HIP(hipGetLastError(), "reduced operators launch");
   // unprojection
   return unproject_with(p, 1, q, velocity, stream);
+}
+
+int cfs_plume_set_params(cfs_plume* p, const cfs_plume_params* prm) {
+  if (!p || !prm) return cfs::fail(CFS_ERR_ARG, "null argument");
+  if (!(prm->dt > 0.0) || !(prm->dx > 0.0)) return cfs::fail(CFS_ERR_ARG, "dt and dx must be positive");
+  p->prm = *prm;
+  return CFS_OK;
+}
+
+int cfs_plume_advection_state(const cfs_plume* p, const double** q_pre, const double** q_adv) {
+  if (!p) return cfs::fail(CFS_ERR_ARG, "null plume");
+  if (q_pre) *q_pre = p->d_qpre;
+  if (q_adv) *q_adv = p->d_q;
+  return CFS_OK;
 }
 
 int cfs_plume_destroy(cfs_plume* p) {

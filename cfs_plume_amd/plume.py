@@ -101,6 +101,18 @@ class PlumeSolver:
         _lib.check(_lib.lib().cfs_plume_set_exact_after(self._h, U_adv.ptr), "cfs_plume_set_exact_after")
         self._after = U_adv
 
+    def set_params(self, params: _lib.PlumeParams):
+        _lib.check(_lib.lib().cfs_plume_set_params(self._h, ctypes.byref(params)), "cfs_plume_set_params")
+        self.params = params
+
+    def advection_state(self):
+        """Views of the last step's advection input (q after the projection,
+        r_pre values) and output (r_adv values) on the device."""
+        a, b = ctypes.c_void_p(), ctypes.c_void_p()
+        _lib.check(_lib.lib().cfs_plume_advection_state(self._h, ctypes.byref(a), ctypes.byref(b)), "advection state")
+        return (DeviceArray.view(a.value, (self.r,), np.float64, self),
+                DeviceArray.view(b.value, (self.r_adv,), np.float64, self))
+
     def fields(self):
         """Zeroed device state: velocity (3 per cell), density, heat, q."""
         n = self.full[0] * self.full[1] * self.full[2]

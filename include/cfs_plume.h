@@ -290,6 +290,9 @@ typedef struct {
   double vorticity_eps;  /* _vorticityEps after its resolution scaling; 0 turns confinement off */
   int open_x, open_y, open_z;  /* 1: open side, the border copies its neighbour
                                   (copyBorderX/Y/Z); 0: wall, the normal component is zeroed */
+  double smoke_offset_x;       /* shift of the smoke column's axis, in cells (0: the reference's source;
+                                  the learned closure's seeded runs move it) */
+  double smoke_offset_z;
 } cfs_plume_params;
 #define CFS_PLUME_ADVECT_EXACT 0
 #define CFS_PLUME_ADVECT_CUBATURE 1
@@ -304,7 +307,9 @@ void cfs_plume_default_params(int X, int Y, int Z, cfs_plume_params* params);
  * r_fin): damping is r_proj x r_adv (damping.peeled), ptof and
  * inverse_product are r_fin x r_proj (projected.ptof, inverseProduct), host
  * FP64 row-major; every dimension <= 256.  The exact advection and the
- * closure stay in U_preadvect coordinates (r_adv = r_pre).  The q of
+ * closure read U_preadvect coordinates; the exact advection writes them too
+ * (r_adv = r_pre), a closure may write more (r_adv >= r_pre: its input is
+ * q zero-padded to r_adv).  The q of
  * cfs_plume_step holds max(r_pre, r_fin) doubles; on return q[0, r_fin) are
  * the final coordinates. */
 int cfs_plume_create(cfs_basis* pre, cfs_basis* fin, int r_adv, int r_proj, const double* damping, const double* ptof,
@@ -327,6 +332,12 @@ int cfs_plume_set_exact_after(cfs_plume* p, const double* U_adv);
 int cfs_plume_set_advection(cfs_plume* p, int mode, cfs_cubature* cubature, cfs_closure* closure);
 int cfs_plume_add_smoke(cfs_plume* p, float* density, float* heat, void* stream);
 int cfs_plume_step(cfs_plume* p, float* velocity, float* density, float* heat, double* q, void* stream);
+/* Replaces the step's parameters (e.g. a run's source offset and buoyancy). */
+int cfs_plume_set_params(cfs_plume* p, const cfs_plume_params* params);
+/* Device pointers to the last step's subspace advection: its input (q after
+ * the projection, r_pre values followed by zeros up to 256) and its output
+ * (r_adv values) -- the pairs a learned closure is trained on. */
+int cfs_plume_advection_state(const cfs_plume* p, const double** q_pre, const double** q_adv);
 int cfs_plume_destroy(cfs_plume* p);
 
 /* ---- the full-rank plume (snapshot source of the reduced bases) --------- */
