@@ -41,6 +41,8 @@ SIGNATURES = {
     "cfs_batch_tune": (_i, [_vp, _i, _vp]),
     "cfs_batch_get_config": (_i, [_vp, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(ctypes.c_float)]),
     "cfs_batch_set_config": (_i, [_vp, _i, ctypes.c_float]),
+    "cfs_batch_get_shape": (_i, [_vp, ctypes.POINTER(_i)]),
+    "cfs_batch_set_shape": (_i, [_vp, _i]),
     "cfs_batch_graph_capture": (_i, [_vp, _vp]),
     "cfs_batch_graph_launch": (_i, [_vp, _vp]),
     "cfs_batch_destroy": (_i, [_vp]),

@@ -266,12 +266,16 @@ class BasisBatch:
         return self.config()
 
     def config(self):
-        g, w, iw = ctypes.c_int(), ctypes.c_int(), ctypes.c_float()
+        g, w, iw, wpw = ctypes.c_int(), ctypes.c_int(), ctypes.c_float(), ctypes.c_int()
         _lib.check(_lib.lib().cfs_batch_get_config(self._h, ctypes.byref(g), ctypes.byref(w), ctypes.byref(iw)))
-        return {"grid": g.value, "wg_per_cu": w.value, "item_weight": iw.value}
+        _lib.check(_lib.lib().cfs_batch_get_shape(self._h, ctypes.byref(wpw)))
+        return {"grid": g.value, "wg_per_cu": w.value, "item_weight": iw.value, "waves_per_wg": wpw.value}
 
     def set_config(self, wg_per_cu: int, item_weight: float):
         _lib.check(_lib.lib().cfs_batch_set_config(self._h, wg_per_cu, item_weight), "cfs_batch_set_config")
+
+    def set_shape(self, waves_per_wg: int):
+        _lib.check(_lib.lib().cfs_batch_set_shape(self._h, waves_per_wg), "cfs_batch_set_shape")
 
     def capture(self, stream: Stream):
         _lib.check(_lib.lib().cfs_batch_graph_capture(self._h, _sptr(stream)), "cfs_batch_graph_capture")

@@ -1,6 +1,7 @@
 // batch.cpp -- several independent compressed bases stepped in one persistent
 // launch, with a HIP-graph capture of the whole step and an autotuner for the
-// launch shape.
+// launch and tile shape (waves per workgroup, workgroups per CU, the work
+// split between bases).
 //
 // The reference steps one basis pair per call (stepPlume,
 // SUBSPACE_FLUID_3D_COMPRESSED_EIGEN.cpp:453-501).  A batch holds bases of
@@ -21,10 +22,10 @@
 #include "cfs_device.h"
 
 namespace cfs {
-hipError_t launch_batch_step(const BatchEntry* d_entries, int n, int mode, int grid, int qrows, size_t lds,
+hipError_t launch_batch_step(const BatchEntry* d_entries, int n, int mode, int grid, int qrows, size_t lds, int wpw,
                              hipStream_t s);
-hipError_t batch_occupancy(int mode, size_t lds, int* wg_per_cu);
-size_t step_lds_bytes_for(int r, int nk);
+hipError_t batch_occupancy(int mode, size_t lds, int wpw, int* wg_per_cu);
+size_t step_lds_bytes_for(int r, int nk, int wpw);
 }  // namespace cfs
 
 using namespace cfs;
@@ -36,8 +37,13 @@ struct cfs_batch {
   BatchEntry* d_entries = nullptr;
   double* d_partials = nullptr;
   size_t partial_doubles = 0;
+  // per tile shape (index 0: 2 waves per workgroup, 1: 4): LDS bytes and
+  // workgroups per CU that fit
+  size_t lds_of[2] = {0, 0};
+  int occupancy_of[2] = {1, 1};
   size_t lds = 0;
   int cus = 0, occupancy = 1;
+  int wpw = 4;           // tuned: waves per workgroup (2 or 4)
   int wg_per_cu = 0;     // tuned: workgroups per CU of the persistent grid
   float item_weight = 0; // tuned: per-item work (in tokens) added to a basis's token count when splitting
   int grid = 0, qrows = 0;
@@ -119,7 +125,11 @@ int upload(cfs_batch* b) {
   return e == hipSuccess ? CFS_OK : hip_fail(e, "hipMemcpy batch entries");
 }
 
-int apply(cfs_batch* b, int wg_per_cu, float item_weight) {
+int apply(cfs_batch* b, int wpw, int wg_per_cu, float item_weight) {
+  const int k = wpw == 2 ? 0 : 1;
+  b->wpw = wpw == 2 ? 2 : 4;
+  b->lds = b->lds_of[k];
+  b->occupancy = b->occupancy_of[k];
   b->wg_per_cu = std::max(1, std::min(wg_per_cu, b->occupancy));
   b->item_weight = item_weight;
   if (!plan(b, std::max((int)b->bases.size(), b->cus * b->wg_per_cu)))
@@ -138,22 +148,22 @@ int cfs_batch_create(cfs_basis* const* bases, int n, cfs_batch** out) {
   std::unique_ptr<cfs_batch> b(new cfs_batch);
   b->device = basis_device(bases[0]);
   int max_r = 0;
-  size_t lds = 0;
   for (int i = 0; i < n; i++) {
     if (!bases[i]) return fail(CFS_ERR_ARG, "null basis in batch");
     if (basis_device(bases[i]) != b->device) return fail(CFS_ERR_ARG, "batch bases must share one device");
     b->bases.push_back(bases[i]);
     const DevBasis& D = basis_dev(bases[i]);
     max_r = std::max(max_r, D.r);
-    lds = std::max(lds, step_lds_bytes_for(D.r, D.nk));
+    for (int k = 0; k < 2; k++) b->lds_of[k] = std::max(b->lds_of[k], step_lds_bytes_for(D.r, D.nk, k ? 4 : 2));
   }
   cfs::DeviceGuard g(b->device);
-  b->lds = lds;
   hipError_t e = hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, b->device);
   if (e != hipSuccess) return hip_fail(e, "hipDeviceGetAttribute");
-  e = batch_occupancy(3, lds, &b->occupancy);
-  if (e != hipSuccess) return hip_fail(e, "hipOccupancyMaxActiveBlocksPerMultiprocessor");
-  if (b->occupancy < 1) return fail(CFS_ERR_ARG, "batch LDS needs exceed one workgroup per CU");
+  for (int k = 0; k < 2; k++) {
+    e = batch_occupancy(3, b->lds_of[k], k ? 4 : 2, &b->occupancy_of[k]);
+    if (e != hipSuccess) return hip_fail(e, "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+    if (b->occupancy_of[k] < 1) return fail(CFS_ERR_ARG, "batch LDS needs exceed one workgroup per CU");
+  }
   b->entries.resize(n);
   for (int i = 0; i < n; i++) {
     BatchEntry& be = b->entries[i];
@@ -162,13 +172,13 @@ int cfs_batch_create(cfs_basis* const* bases, int n, cfs_batch** out) {
     be.damp = basis_damp(bases[i]);
     be.tab = reinterpret_cast<const Tables*>(basis_tables(bases[i]));
   }
-  const int max_grid = std::max(n, b->cus * b->occupancy);
+  const int max_grid = std::max(n, b->cus * std::max(b->occupancy_of[0], b->occupancy_of[1]));
   b->partial_doubles = (size_t)(max_grid + n) * max_r;
   e = hipMalloc(reinterpret_cast<void**>(&b->d_partials), b->partial_doubles * sizeof(double));
   if (e != hipSuccess) return hip_fail(e, "hipMalloc batch partials");
   e = hipMalloc(reinterpret_cast<void**>(&b->d_entries), (size_t)n * sizeof(BatchEntry));
   if (e != hipSuccess) return hip_fail(e, "hipMalloc batch entries");
-  int st = apply(b.get(), b->occupancy, 0.0f);
+  int st = apply(b.get(), 4, b->occupancy_of[1], 0.0f);
   if (st) return st;
   *out = b.release();
   return CFS_OK;
@@ -195,7 +205,7 @@ int cfs_batch_step(cfs_batch* b, void* stream) {
   if (!b->bound) return fail(CFS_ERR_ARG, "cfs_batch_bind must precede cfs_batch_step");
   cfs::DeviceGuard g(b->device);
   hipError_t e = launch_batch_step(b->d_entries, (int)b->entries.size(), 3, b->grid, b->qrows, b->lds,
-                                   (hipStream_t)stream);
+                                   b->wpw, (hipStream_t)stream);
   return e == hipSuccess ? CFS_OK : hip_fail(e, "batch step launch");
 }
 
@@ -211,18 +221,19 @@ int cfs_batch_tune(cfs_batch* b, int reps, void* stream) {
   if (e != hipSuccess) return hip_fail(e, "hipEventCreate");
   const float weights[] = {0.0f, 256.0f, 1024.0f, 4096.0f};
   float best_ms = 1e30f;
-  int best_w = b->occupancy;
+  int best_wpw = 4, best_w = b->occupancy_of[1];
   float best_iw = 0.0f;
   int st = CFS_OK;
-  for (int w = 1; w <= b->occupancy && st == CFS_OK; w++) {
+  for (int wpw : {2, 4})
+  for (int w = 1; w <= b->occupancy_of[wpw == 2 ? 0 : 1] && st == CFS_OK; w++) {
     for (float iw : weights) {
-      st = apply(b, w, iw);
+      st = apply(b, wpw, w, iw);
       if (st) break;
       // one untimed launch, then the timed ones
-      e = launch_batch_step(b->d_entries, (int)b->entries.size(), 3, b->grid, b->qrows, b->lds, s);
+      e = launch_batch_step(b->d_entries, (int)b->entries.size(), 3, b->grid, b->qrows, b->lds, b->wpw, s);
       if (e == hipSuccess) e = hipEventRecord(e0, s);
       for (int i = 0; e == hipSuccess && i < reps; i++)
-        e = launch_batch_step(b->d_entries, (int)b->entries.size(), 3, b->grid, b->qrows, b->lds, s);
+        e = launch_batch_step(b->d_entries, (int)b->entries.size(), 3, b->grid, b->qrows, b->lds, b->wpw, s);
       if (e == hipSuccess) e = hipEventRecord(e1, s);
       if (e == hipSuccess) e = hipEventSynchronize(e1);
       float ms = 0.0f;
@@ -233,6 +244,7 @@ int cfs_batch_tune(cfs_batch* b, int reps, void* stream) {
       }
       if (ms < best_ms) {
         best_ms = ms;
+        best_wpw = wpw;
         best_w = w;
         best_iw = iw;
       }
@@ -241,7 +253,7 @@ int cfs_batch_tune(cfs_batch* b, int reps, void* stream) {
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (st) return st;
-  return apply(b, best_w, best_iw);
+  return apply(b, best_wpw, best_w, best_iw);
 }
 
 int cfs_batch_get_config(const cfs_batch* b, int* grid, int* wg_per_cu, float* item_weight) {
@@ -256,7 +268,20 @@ int cfs_batch_set_config(cfs_batch* b, int wg_per_cu, float item_weight) {
   if (!b) return fail(CFS_ERR_ARG, "null batch");
   if (wg_per_cu < 1 || item_weight < 0) return fail(CFS_ERR_ARG, "bad batch configuration");
   cfs::DeviceGuard g(b->device);
-  return apply(b, wg_per_cu, item_weight);
+  return apply(b, b->wpw, wg_per_cu, item_weight);
+}
+
+int cfs_batch_get_shape(const cfs_batch* b, int* waves_per_wg) {
+  if (!b) return fail(CFS_ERR_ARG, "null batch");
+  if (waves_per_wg) *waves_per_wg = b->wpw;
+  return CFS_OK;
+}
+
+int cfs_batch_set_shape(cfs_batch* b, int waves_per_wg) {
+  if (!b) return fail(CFS_ERR_ARG, "null batch");
+  if (waves_per_wg != 2 && waves_per_wg != 4) return fail(CFS_ERR_ARG, "waves per workgroup must be 2 or 4");
+  cfs::DeviceGuard g(b->device);
+  return apply(b, waves_per_wg, b->wg_per_cu, b->item_weight);
 }
 
 int cfs_batch_graph_capture(cfs_batch* b, void* stream) {
@@ -268,7 +293,7 @@ int cfs_batch_graph_capture(cfs_batch* b, void* stream) {
   hipStream_t s = (hipStream_t)stream;
   hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
   if (e != hipSuccess) return hip_fail(e, "hipStreamBeginCapture");
-  hipError_t le = launch_batch_step(b->d_entries, (int)b->entries.size(), 3, b->grid, b->qrows, b->lds, s);
+  hipError_t le = launch_batch_step(b->d_entries, (int)b->entries.size(), 3, b->grid, b->qrows, b->lds, b->wpw, s);
   e = hipStreamEndCapture(s, &b->graph);
   if (le != hipSuccess) return hip_fail(le, "batch step launch during capture");
   if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");

@@ -52,8 +52,8 @@ __host__ __device__ inline size_t step_wave_lds_bytes(int r) {
 __host__ __device__ inline size_t step_shared_lds_bytes(int nk) {
   return ((size_t)nk * NDIAG * 4 + 15) / 16 * 16;
 }
-__host__ inline size_t step_lds_bytes(int r, int nk) {
-  return step_shared_lds_bytes(nk) + WAVES_PER_WG * step_wave_lds_bytes(r);
+__host__ inline size_t step_lds_bytes(int r, int nk, int wpw = WAVES_PER_WG) {
+  return step_shared_lds_bytes(nk) + (size_t)wpw * step_wave_lds_bytes(r);
 }
 
 __device__ __forceinline__ WaveLds carve_wave(char* base, int r) {
@@ -166,8 +166,9 @@ __device__ __forceinline__ void inverse_dct_block_mfma(float* __restrict__ field
   }
 }
 
-// The fused step of one basis, run by workgroup `wg` of `nwg` that share it.
-template <int MODE>
+// The fused step of one basis, run by workgroup `wg` of `nwg` that share it;
+// WPW waves per workgroup.
+template <int MODE, int WPW>
 __device__ __forceinline__ void step_body(const DevBasis& B, const float* __restrict__ v_in,
                                           const double* __restrict__ q_in, float* __restrict__ field_out,
                                           const float* __restrict__ g_damp, const Tables* __restrict__ g_tab,
@@ -187,8 +188,8 @@ __device__ __forceinline__ void step_body(const DevBasis& B, const float* __rest
   const WaveLds w = carve_wave(lds + step_shared_lds_bytes(nk) + (size_t)wave * step_wave_lds_bytes(B.r), B.r);
   const int r = B.r;
   const int total = 3 * B.nB;
-  const int nwaves = nwg * WAVES_PER_WG;
-  int item = wg * WAVES_PER_WG + wave;
+  const int nwaves = nwg * WPW;
+  int item = wg * WPW + wave;
 
   ItemPrefetch nx;
   if (item < total) prefetch_item<PROJ>(B, v_in, item, nx);
@@ -374,7 +375,7 @@ __device__ __forceinline__ void step_body(const DevBasis& B, const float* __rest
     __syncthreads();
     for (int c = threadIdx.x; c < r; c += blockDim.x) {
       double s = 0.0;
-      for (int wv = 0; wv < WAVES_PER_WG; wv++)
+      for (int wv = 0; wv < WPW; wv++)
         s += (double)carve_wave(lds + step_shared_lds_bytes(nk) + (size_t)wv * step_wave_lds_bytes(r), r).qacc[c];
       partials[(size_t)wg * r + c] = s;
     }
@@ -385,21 +386,23 @@ template <int MODE>
 __global__ void __launch_bounds__(WAVES_PER_WG * WAVE, 3)  // <= 168 VGPRs: three waves per SIMD
 step_kernel(DevBasis B, const float* __restrict__ v_in, const double* __restrict__ q_in,
             float* __restrict__ field_out, const float* __restrict__ g_damp, const Tables* __restrict__ g_tab) {
-  step_body<MODE>(B, v_in, q_in, field_out, g_damp, g_tab, B.partials, blockIdx.x, gridDim.x);
+  step_body<MODE, WAVES_PER_WG>(B, v_in, q_in, field_out, g_damp, g_tab, B.partials, blockIdx.x, gridDim.x);
 }
 
 // Several independent bases in one persistent launch: workgroups
 // [wg_begin, wg_begin + wg_count) of the grid step entry e (entries sorted by
 // wg_begin, contiguous).
-template <int MODE>
-__global__ void __launch_bounds__(WAVES_PER_WG * WAVE, 3)
+// WPW waves per workgroup (a tile shape of the batch autotuner); 12 / WPW
+// workgroups per CU keep three waves per SIMD within the register budget.
+template <int MODE, int WPW>
+__global__ void __launch_bounds__(WPW * WAVE, 12 / WPW)
 batch_step_kernel(const BatchEntry* __restrict__ E, int n) {
   int e = 0;
   while (e + 1 < n && (int)blockIdx.x >= E[e + 1].wg_begin) e++;
   const BatchEntry& be = E[e];
   const DevBasis B = be.B;
-  step_body<MODE>(B, be.v_in, be.q_in, be.field, be.damp, be.tab, be.partials, (int)blockIdx.x - be.wg_begin,
-                  be.wg_count);
+  step_body<MODE, WPW>(B, be.v_in, be.q_in, be.field, be.damp, be.tab, be.partials, (int)blockIdx.x - be.wg_begin,
+                       be.wg_count);
 }
 
 // q of every entry: block (e, c) sums partials[0..wg_count)[c] of entry e.
@@ -493,41 +496,51 @@ hipError_t step_occupancy(int mode, int r, int nk, int* wg_per_cu) {
   return e;
 }
 
-hipError_t launch_batch_step(const BatchEntry* d_entries, int n, int mode, int grid, int qrows, size_t lds,
-                             hipStream_t s) {
+template <int WPW>
+hipError_t launch_batch_step_w(const BatchEntry* d_entries, int n, int mode, int grid, size_t lds, hipStream_t s) {
   switch (mode) {
     case MODE_PROJECT:
-      hipLaunchKernelGGL(batch_step_kernel<MODE_PROJECT>, dim3(grid), dim3(WAVES_PER_WG * WAVE), lds, s, d_entries, n);
+      hipLaunchKernelGGL((batch_step_kernel<MODE_PROJECT, WPW>), dim3(grid), dim3(WPW * WAVE), lds, s, d_entries, n);
       break;
     case MODE_UNPROJECT:
-      hipLaunchKernelGGL(batch_step_kernel<MODE_UNPROJECT>, dim3(grid), dim3(WAVES_PER_WG * WAVE), lds, s, d_entries,
-                         n);
+      hipLaunchKernelGGL((batch_step_kernel<MODE_UNPROJECT, WPW>), dim3(grid), dim3(WPW * WAVE), lds, s, d_entries, n);
       break;
     default:
-      hipLaunchKernelGGL(batch_step_kernel<MODE_BOTH>, dim3(grid), dim3(WAVES_PER_WG * WAVE), lds, s, d_entries, n);
+      hipLaunchKernelGGL((batch_step_kernel<MODE_BOTH, WPW>), dim3(grid), dim3(WPW * WAVE), lds, s, d_entries, n);
       break;
   }
-  hipError_t e = hipGetLastError();
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_step(const BatchEntry* d_entries, int n, int mode, int grid, int qrows, size_t lds, int wpw,
+                             hipStream_t s) {
+  hipError_t e = wpw == 2 ? launch_batch_step_w<2>(d_entries, n, mode, grid, lds, s)
+                          : launch_batch_step_w<4>(d_entries, n, mode, grid, lds, s);
   if (e != hipSuccess || !(mode & MODE_PROJECT)) return e;
   hipLaunchKernelGGL(batch_reduce_kernel, dim3(qrows), dim3(256), 0, s, d_entries, n);
   return hipGetLastError();
 }
 
-hipError_t batch_occupancy(int mode, size_t lds, int* wg_per_cu) {
+template <int WPW>
+hipError_t batch_occupancy_w(int mode, size_t lds, int* wg_per_cu) {
   switch (mode) {
     case MODE_PROJECT:
-      return hipOccupancyMaxActiveBlocksPerMultiprocessor(wg_per_cu, batch_step_kernel<MODE_PROJECT>,
-                                                          WAVES_PER_WG * WAVE, lds);
+      return hipOccupancyMaxActiveBlocksPerMultiprocessor(wg_per_cu, batch_step_kernel<MODE_PROJECT, WPW>, WPW * WAVE,
+                                                          lds);
     case MODE_UNPROJECT:
-      return hipOccupancyMaxActiveBlocksPerMultiprocessor(wg_per_cu, batch_step_kernel<MODE_UNPROJECT>,
-                                                          WAVES_PER_WG * WAVE, lds);
+      return hipOccupancyMaxActiveBlocksPerMultiprocessor(wg_per_cu, batch_step_kernel<MODE_UNPROJECT, WPW>,
+                                                          WPW * WAVE, lds);
     default:
-      return hipOccupancyMaxActiveBlocksPerMultiprocessor(wg_per_cu, batch_step_kernel<MODE_BOTH>,
-                                                          WAVES_PER_WG * WAVE, lds);
+      return hipOccupancyMaxActiveBlocksPerMultiprocessor(wg_per_cu, batch_step_kernel<MODE_BOTH, WPW>, WPW * WAVE,
+                                                          lds);
   }
 }
 
-size_t step_lds_bytes_for(int r, int nk) { return step_lds_bytes(r, nk); }
+hipError_t batch_occupancy(int mode, size_t lds, int wpw, int* wg_per_cu) {
+  return wpw == 2 ? batch_occupancy_w<2>(mode, lds, wg_per_cu) : batch_occupancy_w<4>(mode, lds, wg_per_cu);
+}
+
+size_t step_lds_bytes_for(int r, int nk, int wpw) { return step_lds_bytes(r, nk, wpw); }
 
 size_t tables_bytes() { return sizeof(Tables); }
 

@@ -139,11 +139,17 @@ int cfs_batch_create(cfs_basis* const* bases, int n, cfs_batch** out);
 int cfs_batch_bind(cfs_batch* batch, const float* const* v_in, double* const* q_out, const double* const* q_in,
                    float* const* field);
 int cfs_batch_step(cfs_batch* batch, void* stream);
-/* Autotuner: times every launch shape (workgroups per CU x work-split weight)
- * `reps` times on the bound arrays and keeps the fastest. */
+/* Autotuner: times every tile and launch shape (waves per workgroup 2 or 4
+ * x workgroups per CU x work-split weight) `reps` times on the bound arrays
+ * and keeps the fastest. */
 int cfs_batch_tune(cfs_batch* batch, int reps, void* stream);
 int cfs_batch_get_config(const cfs_batch* batch, int* grid, int* wg_per_cu, float* item_weight);
 int cfs_batch_set_config(cfs_batch* batch, int wg_per_cu, float item_weight);
+/* The tile shape: waves per workgroup (2 or 4; each wave takes one
+ * (component, block) item at a time, the workgroup shares the damping
+ * table, so two-wave workgroups halve the LDS per workgroup). */
+int cfs_batch_get_shape(const cfs_batch* batch, int* waves_per_wg);
+int cfs_batch_set_shape(cfs_batch* batch, int waves_per_wg);
 /* Captures one whole batched step on `stream` (not the default stream) into
  * a HIP graph held by the batch; cfs_batch_graph_launch replays it.  Binding,
  * tuning or setting the configuration drops the graph. */

@@ -90,9 +90,33 @@ def test_batch_autotuner(bases):
     ref = _reference(bases)
     bt, arrs = _batch(bases)
     cfg = bt.tune(reps=2)
-    assert cfg["wg_per_cu"] >= 1 and cfg["grid"] >= len(bases)
+    assert cfg["wg_per_cu"] >= 1 and cfg["grid"] >= len(bases) and cfg["waves_per_wg"] in (2, 4)
+    print(f"[batch] tuned {cfg}", flush=True)
+    bt.set_shape(4)
     bt.step()
     _check(bases, arrs, ref)
+    bt.close()
+
+
+def test_batch_tile_shapes(bases):
+    """Two-wave workgroups: the same field bit for bit (unproject does not
+    depend on how items are grouped), q within 1e-6 of the basis's own step
+    (the FP32 per-wave sums are grouped differently) and within the oracle
+    tolerance."""
+    ref = _reference(bases)
+    bt, arrs = _batch(bases)
+    for w in (1, 3):
+        bt.set_shape(2)
+        bt.set_config(w, 256.0)
+        assert bt.config()["waves_per_wg"] == 2
+        for a in arrs:
+            a[5].fill_zero()
+        bt.step()
+        for (ob, gb), a, (q1, f1, qo_want, fo_want) in zip(bases, arrs, ref):
+            q, f = a[4].numpy(), a[5].numpy()
+            assert np.array_equal(f, f1)
+            assert rel(q, q1) <= 1e-6
+            assert rel(q, qo_want) <= TOL and rel(f, fo_want) <= TOL
     bt.close()
 
 
