@@ -30,6 +30,7 @@ SIGNATURES = {
     "cfs_unproject": (_i, [_vp, _vp, _vp, _vp]),
     "cfs_step": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "cfs_decode_columns": (_i, [_vp, _i, _i, _vp, _vp]),
+    "cfs_unproject_spatial": (_i, [_vp, _vp, _vp, _vp, _i, _vp]),
     "cfs_untransform_svd": (_i, [_vp, _i, ctypes.c_int64, _vp, _vp]),
     "cfs_decode_block_ints": (_i, [_vp, _i, _i, _i, _i, _vp, _vp]),
     "cfs_decode_block_coeffs": (_i, [_vp, _i, _i, _i, _i, _vp, _vp]),

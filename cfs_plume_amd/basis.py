@@ -164,6 +164,20 @@ class CompressedBasis:
         _lib.check(_lib.lib().cfs_unproject(self._h, q.ptr, field.ptr, _sptr(stream)), "unproject")
         return field
 
+    def unproject_spatial(self, q: DeviceArray, field: DeviceArray, group: int = 8,
+                          scratch: Optional[DeviceArray] = None, stream: Optional[Stream] = None) -> DeviceArray:
+        """PeeledCompressedUnproject (COMPRESSION.cpp:2677-2711): the columns
+        decoded spatially `group` at a time and summed with weights q into the
+        interior of the full-grid field (boundary cells untouched)."""
+        assert q.dtype == np.float64 and q.nbytes == 8 * self.r
+        assert field.dtype == np.float32 and field.nbytes == 12 * self.full_cells
+        g = min(group, self.r)
+        if scratch is not None:
+            assert scratch.dtype == np.float32 and scratch.nbytes >= 12 * g * self.peeled_cells
+        _lib.check(_lib.lib().cfs_unproject_spatial(self._h, q.ptr, field.ptr, scratch.ptr if scratch else None,
+                                                    group, _sptr(stream)), "PeeledCompressedUnproject")
+        return field
+
     def decode_columns(self, col0: int = 0, ncols: Optional[int] = None, out: Optional[DeviceArray] = None,
                        stream: Optional[Stream] = None) -> DeviceArray:
         ncols = self.r - col0 if ncols is None else ncols

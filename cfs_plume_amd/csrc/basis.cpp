@@ -38,6 +38,8 @@ void fill_mfma_frags(void* host_tab);
 hipError_t launch_decode(const DevBasis& B, int mode, const DecodeArgs& a, const float* d_damp, const void* d_tab,
                          hipStream_t s);
 hipError_t launch_untransform_svd(float* cols, long cells, int ncols, const double* V, hipStream_t s);
+hipError_t launch_accumulate_columns(const float* cols, int ncols, long n3, const double* q, float* field, int X,
+                                     int Y, int Z, int first, hipStream_t s);
 }  // namespace cfs
 
 using namespace cfs;
@@ -689,6 +691,33 @@ int cfs_get_submatrix(cfs_basis* B, int start_row, double* out, void* stream) {
   const int nbx = B->Px / BS, nby = B->Py / BS;
   const int blk = x / BS + (y / BS) * nbx + (z / BS) * nbx * nby;
   return decode_common(B, 3, 0, 3, 0, B->r, blk, 1, point, out, stream);
+}
+
+int cfs_unproject_spatial(cfs_basis* B, const double* q, float* field, float* scratch, int group, void* stream) {
+  if (!B || !q || !field) return fail(CFS_ERR_ARG, "null argument");
+  if (group < 1) return fail(CFS_ERR_ARG, "group must be at least one column");
+  const DevBasis& D = B->dev;
+  const int r = D.r;
+  group = std::min(group, r);
+  const long n3 = 3L * D.X * D.Y * D.Z;
+  DeviceGuard g(B->device);
+  const hipStream_t s = (hipStream_t)stream;
+  float* buf = scratch;
+  if (!buf) {
+    const hipError_t e = hipMallocAsync((void**)&buf, (size_t)group * n3 * sizeof(float), s);
+    if (e != hipSuccess) return hip_fail(e, "hipMallocAsync");
+  }
+  int st = CFS_OK;
+  for (int c0 = 0; c0 < r && st == CFS_OK; c0 += group) {
+    const int n = std::min(group, r - c0);
+    st = cfs_decode_columns(B, c0, n, buf, stream);
+    if (st == CFS_OK) {
+      const hipError_t e = launch_accumulate_columns(buf, n, n3, q + c0, field, D.X, D.Y, D.Z, c0 == 0, s);
+      if (e != hipSuccess) st = hip_fail(e, "accumulate launch");
+    }
+  }
+  if (!scratch) (void)hipFreeAsync(buf, s);
+  return st;
 }
 
 int cfs_untransform_svd(float* columns, int ncols, int64_t cells, const double* V, void* stream) {

@@ -392,6 +392,34 @@ hipError_t launch_untransform_svd(float* cols, long cells, int ncols, const doub
   return hipGetLastError();
 }
 
+// PeeledCompressedUnproject's accumulation (COMPRESSION.cpp:2677-2711) for a
+// group of decoded columns: the interior of the full-grid field gets
+// sum_k q[k] cols[k] (FP64 sums, added to the field's value unless `first`).
+__global__ void __launch_bounds__(256) accumulate_columns_kernel(const float* __restrict__ cols, int ncols, long n3,
+                                                                 const double* __restrict__ q,
+                                                                 float* __restrict__ field, int X, int Y, int Z,
+                                                                 int first) {
+  const long j = (long)blockIdx.x * 256 + threadIdx.x;
+  if (j >= n3) return;
+  const long cell = j / 3;
+  const int comp = (int)(j % 3);
+  const int x = (int)(cell % X), y = (int)((cell / X) % Y), z = (int)(cell / ((long)X * Y));
+  const long f = 3 * ((long)(x + 1) + (long)(X + 2) * ((y + 1) + (long)(Y + 2) * (z + 1))) + comp;
+  double acc = 0.0;
+  for (int k = 0; k < ncols; k++) acc += q[k] * (double)cols[(long)k * n3 + j];
+  field[f] = first ? (float)acc : (float)((double)field[f] + acc);
+}
+
+hipError_t launch_accumulate_columns(const float* cols, int ncols, long n3, const double* q, float* field, int X,
+                                     int Y, int Z, int first, hipStream_t s) {
+  if (n3 <= 0 || ncols <= 0) return hipSuccess;
+  const long blocks = (n3 + 255) / 256;
+  if (blocks > 0x7fffffffL) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(accumulate_columns_kernel, dim3((unsigned)blocks), dim3(256), 0, s, cols, ncols, n3, q, field,
+                     X, Y, Z, first);
+  return hipGetLastError();
+}
+
 // Host: the MFMA operands of the inverse DCT in lane order (FP64 closed form
 // split into BF16 hi + lo, round to nearest even).
 static uint16_t host_bf16_bits(float x) {

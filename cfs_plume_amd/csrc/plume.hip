@@ -108,7 +108,7 @@ __global__ void __launch_bounds__(MAX_R) reduced_ops_kernel(const double* __rest
 
 }  // namespace
 
-struct cfs_plume {
+struct cfs_plume_solver {
   int device = 0;
   cfs_basis* pre = nullptr;
   cfs_basis* fin = nullptr;
@@ -129,7 +129,7 @@ struct cfs_plume {
   double* d_col = nullptr;  // an FP64 peeled vector field
   float *d_w = nullptr, *d_wmag = nullptr, *d_dold = nullptr, *d_hold = nullptr, *d_fa = nullptr, *d_fb = nullptr;
   std::vector<void*> allocs;
-  ~cfs_plume() {
+  ~cfs_plume_solver() {
     for (void* p : allocs) (void)hipFree(p);
   }
 };
@@ -137,7 +137,7 @@ struct cfs_plume {
 namespace {
 
 template <class T>
-int dev_alloc(cfs_plume* p, T** out, size_t count, const T* host) {
+int dev_alloc(cfs_plume_solver* p, T** out, size_t count, const T* host) {
   const hipError_t e = hipMalloc(reinterpret_cast<void**>(out), std::max<size_t>(count, 1) * sizeof(T));
   if (e != hipSuccess) return cfs::hip_fail(e, "hipMalloc");
   p->allocs.push_back(*out);
@@ -153,10 +153,10 @@ int dev_alloc(cfs_plume* p, T** out, size_t count, const T* host) {
     if (e_ != hipSuccess) return cfs::hip_fail(e_, what);   \
   } while (0)
 
-long peeled(const cfs_plume* p) { return (long)(p->X - 2) * (p->Y - 2) * (p->Z - 2); }
+long peeled(const cfs_plume_solver* p) { return (long)(p->X - 2) * (p->Y - 2) * (p->Z - 2); }
 
 // q = U^T v with U_preadvect, compressed or dense
-int project_pre(cfs_plume* p, const float* field, double* q, void* stream) {
+int project_pre(cfs_plume_solver* p, const float* field, double* q, void* stream) {
   if (!p->U_pre_dense) return cfs_project(p->pre, field, q, stream);
   const hipStream_t s = (hipStream_t)stream;
   const long np = peeled(p);
@@ -167,7 +167,7 @@ int project_pre(cfs_plume* p, const float* field, double* q, void* stream) {
 }
 
 // field interior = U q, with U_preadvect (which = 0) or U_final (which = 1)
-int unproject_with(cfs_plume* p, int which, const double* q, float* field, void* stream) {
+int unproject_with(cfs_plume_solver* p, int which, const double* q, float* field, void* stream) {
   const double* U = which ? p->U_fin_dense : p->U_pre_dense;
   if (!U) return cfs_unproject(which ? p->fin : p->pre, q, field, stream);
   const hipStream_t s = (hipStream_t)stream;
@@ -208,7 +208,7 @@ void cfs_plume_default_params(int X, int Y, int Z, cfs_plume_params* prm) {
 namespace {
 
 // the reduced matrices and the scratch of a plume whose grid and dimensions are set
-int finish_plume(cfs_plume* P, const double* damping, const double* ptof, const double* inverse_product,
+int finish_plume(cfs_plume_solver* P, const double* damping, const double* ptof, const double* inverse_product,
                  const cfs_plume_params* prm) {
   if (P->r_pre < 1 || P->r_pre > MAX_R || P->r_fin < 1 || P->r_fin > MAX_R || P->r_adv < 1 || P->r_adv > MAX_R ||
       P->r_proj < 1 || P->r_proj > MAX_R)
@@ -239,7 +239,7 @@ extern "C" {
 
 int cfs_plume_create(cfs_basis* pre, cfs_basis* fin, int r_adv, int r_proj, const double* damping,
                      const double* ptof, const double* inverse_product, const cfs_plume_params* prm,
-                     cfs_plume** out) {
+                     cfs_plume_solver** out) {
   if (!out) return cfs::fail(CFS_ERR_ARG, "null output handle");
   *out = nullptr;
   if (!pre || !fin || !damping || !ptof || !inverse_product || !prm) return cfs::fail(CFS_ERR_ARG, "null argument");
@@ -249,7 +249,7 @@ int cfs_plume_create(cfs_basis* pre, cfs_basis* fin, int r_adv, int r_proj, cons
   if (a.dims[0] != b.dims[0] || a.dims[1] != b.dims[1] || a.dims[2] != b.dims[2])
     return cfs::fail(CFS_ERR_ARG, "U_preadvect and U_final must share the grid");
   if (cfs::basis_device(pre) != cfs::basis_device(fin)) return cfs::fail(CFS_ERR_ARG, "bases on different devices");
-  std::unique_ptr<cfs_plume> P(new cfs_plume);
+  std::unique_ptr<cfs_plume_solver> P(new cfs_plume_solver);
   P->device = cfs::basis_device(pre);
   P->pre = pre;
   P->fin = fin;
@@ -267,13 +267,13 @@ int cfs_plume_create(cfs_basis* pre, cfs_basis* fin, int r_adv, int r_proj, cons
 
 int cfs_plume_create_dense(const double* U_pre, int r_pre, const double* U_fin, int r_fin, int X, int Y, int Z,
                            int r_adv, int r_proj, const double* damping, const double* ptof,
-                           const double* inverse_product, const cfs_plume_params* prm, cfs_plume** out) {
+                           const double* inverse_product, const cfs_plume_params* prm, cfs_plume_solver** out) {
   if (!out) return cfs::fail(CFS_ERR_ARG, "null output handle");
   *out = nullptr;
   if (!U_pre || !U_fin || !damping || !ptof || !inverse_product || !prm)
     return cfs::fail(CFS_ERR_ARG, "null argument");
   if (X < 2 || Y < 2 || Z < 2) return cfs::fail(CFS_ERR_ARG, "bad peeled grid");
-  std::unique_ptr<cfs_plume> P(new cfs_plume);
+  std::unique_ptr<cfs_plume_solver> P(new cfs_plume_solver);
   PL_HIP(hipGetDevice(&P->device), "hipGetDevice");
   P->U_pre_dense = U_pre;
   P->U_fin_dense = U_fin;
@@ -290,14 +290,14 @@ int cfs_plume_create_dense(const double* U_pre, int r_pre, const double* U_fin, 
   return CFS_OK;
 }
 
-int cfs_plume_set_exact_after(cfs_plume* p, const double* U_adv) {
+int cfs_plume_set_exact_after(cfs_plume_solver* p, const double* U_adv) {
   if (!p || !U_adv) return cfs::fail(CFS_ERR_ARG, "null argument");
   if (!p->U_pre_dense) return cfs::fail(CFS_ERR_ARG, "the exact after-basis needs a dense plume");
   p->U_adv_dense = U_adv;
   return CFS_OK;
 }
 
-int cfs_plume_set_advection(cfs_plume* p, int mode, cfs_cubature* cub, cfs_closure* closure) {
+int cfs_plume_set_advection(cfs_plume_solver* p, int mode, cfs_cubature* cub, cfs_closure* closure) {
   if (!p) return cfs::fail(CFS_ERR_ARG, "null plume");
   if (mode == CFS_PLUME_ADVECT_CUBATURE) {
     int K = 0, r = 0;
@@ -325,7 +325,7 @@ int cfs_plume_set_advection(cfs_plume* p, int mode, cfs_cubature* cub, cfs_closu
   return CFS_OK;
 }
 
-int cfs_plume_add_smoke(cfs_plume* p, float* density, float* heat, void* stream) {
+int cfs_plume_add_smoke(cfs_plume_solver* p, float* density, float* heat, void* stream) {
   if (!p || !density || !heat) return cfs::fail(CFS_ERR_ARG, "null argument");
   cfs::DeviceGuard g(p->device);
   const int n = p->X * p->Y * p->Z;
@@ -336,7 +336,7 @@ int cfs_plume_add_smoke(cfs_plume* p, float* density, float* heat, void* stream)
   return CFS_OK;
 }
 
-int cfs_plume_step(cfs_plume* p, float* velocity, float* density, float* heat, double* q, void* stream) {
+int cfs_plume_step(cfs_plume_solver* p, float* velocity, float* density, float* heat, double* q, void* stream) {
   if (!p || !velocity || !density || !heat || !q) return cfs::fail(CFS_ERR_ARG, "null argument");
   if (p->mode == CFS_PLUME_ADVECT_EXACT && p->r_adv != p->r_pre && !p->U_adv_dense)
     return cfs::fail(CFS_ERR_ARG, "no advection set: the exact advection needs r_adv equal to U_preadvect's r");
@@ -406,21 +406,21 @@ int cfs_plume_step(cfs_plume* p, float* velocity, float* density, float* heat, d
   return unproject_with(p, 1, q, velocity, stream);
 }
 
-int cfs_plume_set_params(cfs_plume* p, const cfs_plume_params* prm) {
+int cfs_plume_set_params(cfs_plume_solver* p, const cfs_plume_params* prm) {
   if (!p || !prm) return cfs::fail(CFS_ERR_ARG, "null argument");
   if (!(prm->dt > 0.0) || !(prm->dx > 0.0)) return cfs::fail(CFS_ERR_ARG, "dt and dx must be positive");
   p->prm = *prm;
   return CFS_OK;
 }
 
-int cfs_plume_advection_state(const cfs_plume* p, const double** q_pre, const double** q_adv) {
+int cfs_plume_advection_state(const cfs_plume_solver* p, const double** q_pre, const double** q_adv) {
   if (!p) return cfs::fail(CFS_ERR_ARG, "null plume");
   if (q_pre) *q_pre = p->d_qpre;
   if (q_adv) *q_adv = p->d_q;
   return CFS_OK;
 }
 
-int cfs_plume_destroy(cfs_plume* p) {
+int cfs_plume_destroy(cfs_plume_solver* p) {
   if (!p) return cfs::fail(CFS_ERR_ARG, "null plume");
   cfs::DeviceGuard g(p->device);
   delete p;

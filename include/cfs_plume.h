@@ -96,6 +96,14 @@ int cfs_step(cfs_basis* pre, cfs_basis* fin, const float* v_in, double* q_out, c
  * and unpadded, interleaved xyz (the layout of one column of U). */
 int cfs_decode_columns(cfs_basis* basis, int col0, int ncols, float* out, void* stream);
 
+/* PeeledCompressedUnproject (COMPRESSION.cpp:2677-2711): the spatial variant
+ * of cfs_unproject -- every column decoded spatially (DecodeVectorField) and
+ * the interior of field set to sum_c q_c U_c (FP64 sums within a group of
+ * columns, FP32 between groups); boundary cells untouched.  Columns are
+ * decoded `group` at a time into `scratch` (device, group * 3 X Y Z floats;
+ * null: allocated on the stream for the call).  q is device FP64[r]. */
+int cfs_unproject_spatial(cfs_basis* basis, const double* q, float* field, float* scratch, int group, void* stream);
+
 /* UntransformVectorFieldSVD (COMPRESSION.cpp:713-727) applied to ncols
  * decoded columns in place: columns holds ncols x 3*cells FP32 values in the
  * layout of cfs_decode_columns, V (device) holds ncols row-major 3 x 3 FP64
@@ -288,7 +296,7 @@ int cfs_closure_step(cfs_closure* c, const double* q_in, double* q_out, int n, d
  * cfs_plume_add_smoke, addSmokeColumn FLUID_3D_MIC.cpp:655-703) in a HIP
  * graph.  Fields are the caller's full grids (velocity interleaved xyz,
  * density and heat scalar, FP32); q is FP64[r]. */
-typedef struct cfs_plume cfs_plume;
+typedef struct cfs_plume_solver cfs_plume_solver;
 typedef struct {
   double dt;             /* time step (_dt) */
   double dx;             /* cell size (_dx = 1 / max resolution) */
@@ -319,7 +327,7 @@ void cfs_plume_default_params(int X, int Y, int Z, cfs_plume_params* params);
  * cfs_plume_step holds max(r_pre, r_fin) doubles; on return q[0, r_fin) are
  * the final coordinates. */
 int cfs_plume_create(cfs_basis* pre, cfs_basis* fin, int r_adv, int r_proj, const double* damping, const double* ptof,
-                     const double* inverse_product, const cfs_plume_params* params, cfs_plume** out);
+                     const double* inverse_product, const cfs_plume_params* params, cfs_plume_solver** out);
 /* The uncompressed solver (SUBSPACE_FLUID_3D_EIGEN::stepPlume, the accuracy
  * harness's ground truth): the same step with dense bases, device FP64
  * 3 Npeel x r column-major (one mode per column, components interleaved),
@@ -328,23 +336,23 @@ int cfs_plume_create(cfs_basis* pre, cfs_basis* fin, int r_adv, int r_proj, cons
  * on U_pre q), the closure or the exact advection. */
 int cfs_plume_create_dense(const double* U_pre, int r_pre, const double* U_fin, int r_fin, int X, int Y, int Z,
                            int r_adv, int r_proj, const double* damping, const double* ptof,
-                           const double* inverse_product, const cfs_plume_params* params, cfs_plume** out);
+                           const double* inverse_product, const cfs_plume_params* params, cfs_plume_solver** out);
 /* Dense plumes only: the exact advection projects the advected field onto
  * this dense after-basis (device FP64, 3 Npeel x r_adv column-major, e.g.
  * U_prediffuse) instead of U_preadvect, q_adv = U_adv^T A(U_pre q) -- the
  * quantity the cubature approximates. */
-int cfs_plume_set_exact_after(cfs_plume* p, const double* U_adv);
+int cfs_plume_set_exact_after(cfs_plume_solver* p, const double* U_adv);
 /* Selects the subspace advection; the cubature or closure must outlive the plume. */
-int cfs_plume_set_advection(cfs_plume* p, int mode, cfs_cubature* cubature, cfs_closure* closure);
-int cfs_plume_add_smoke(cfs_plume* p, float* density, float* heat, void* stream);
-int cfs_plume_step(cfs_plume* p, float* velocity, float* density, float* heat, double* q, void* stream);
+int cfs_plume_set_advection(cfs_plume_solver* p, int mode, cfs_cubature* cubature, cfs_closure* closure);
+int cfs_plume_add_smoke(cfs_plume_solver* p, float* density, float* heat, void* stream);
+int cfs_plume_step(cfs_plume_solver* p, float* velocity, float* density, float* heat, double* q, void* stream);
 /* Replaces the step's parameters (e.g. a run's source offset and buoyancy). */
-int cfs_plume_set_params(cfs_plume* p, const cfs_plume_params* params);
+int cfs_plume_set_params(cfs_plume_solver* p, const cfs_plume_params* params);
 /* Device pointers to the last step's subspace advection: its input (q after
  * the projection, r_pre values followed by zeros up to 256) and its output
  * (r_adv values) -- the pairs a learned closure is trained on. */
-int cfs_plume_advection_state(const cfs_plume* p, const double** q_pre, const double** q_adv);
-int cfs_plume_destroy(cfs_plume* p);
+int cfs_plume_advection_state(const cfs_plume_solver* p, const double** q_pre, const double** q_adv);
+int cfs_plume_destroy(cfs_plume_solver* p);
 
 /* ---- the full-rank plume (snapshot source of the reduced bases) --------- */
 /* FLUID_3D_MIC (src/integrators/FLUID_3D_MIC.cpp) on the device in FP64:

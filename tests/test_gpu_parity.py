@@ -263,6 +263,33 @@ def test_unproject_keeps_boundary(case, request):
 
 
 @pytest.mark.parametrize("case", CASES)
+def test_unproject_spatial(case, request):
+    """PeeledCompressedUnproject: spatial decode of every column, then the
+    weighted sum; equal to the oracle's unproject and to the fused one."""
+    ob, gb = request.getfixturevalue(case)
+    _, q = _fields(ob, 45)
+    X, Y, Z = ob.dims
+    init = np.full((Z + 2) * (Y + 2) * (X + 2) * 3, -3.5)
+    want = ob.unproject(q, init).reshape(Z + 2, Y + 2, X + 2, 3)
+    dq = DeviceArray.from_numpy(q)
+    fused = gb.unproject(dq, DeviceArray.from_numpy(init.astype(np.float32))).numpy().reshape(want.shape)
+    inner = np.zeros(want.shape, bool)
+    inner[1:-1, 1:-1, 1:-1] = True
+    scratch = DeviceArray((3, 3 * gb.peeled_cells), np.float32)
+    for group, buf in ((1, None), (3, scratch), (ob.r, None)):
+        df = DeviceArray.from_numpy(init.astype(np.float32))
+        got = gb.unproject_spatial(dq, df, group=group, scratch=buf).numpy().astype(np.float64).reshape(want.shape)
+        e = rel(got[inner], want[inner])
+        print(f"[{case}] unproject_spatial group {group}: rel {e:.2e} vs oracle, "
+              f"{rel(got[inner], fused[inner]):.2e} vs fused", flush=True)
+        assert e <= SPATIAL_TOL, (group, e)
+        assert rel(got[inner], fused[inner]) <= SPATIAL_TOL
+        assert np.all(got[~inner] == -3.5), "spatial unproject wrote boundary cells"
+    scratch.free()
+    assert gb.check_stream_errors() == 0
+
+
+@pytest.mark.parametrize("case", CASES)
 def test_fused_step_matches_separate(case, request):
     ob, gb = request.getfixturevalue(case)
     v, q = _fields(ob, 44)
