@@ -39,9 +39,17 @@ hipError_t launch_zero_words(void* p, long nwords, hipStream_t s);
 hipError_t launch_axpb(double* q, int r, double alpha, const double* forcing, hipStream_t s);
 hipError_t launch_mlp_fwd(const float* X, int ldx, int n, const uint16_t* Wp, const float* bias, int K, int M,
                           float* Z, float* Y, int act, hipStream_t s);
-hipError_t launch_gemm_f32(const float* A, long sam, long sak, const float* B, long sbk, long sbn, float* C, int ldc,
-                           int M, int N, int K, const float* G, hipStream_t s);
-hipError_t launch_bias_grad(const float* dZ, int n, int M, float* db, hipStream_t s);
+struct GemmProblem {
+  const float* A;
+  long sam, sak;
+  const float* B;
+  long sbk, sbn;
+  float* C;
+  const float* G;
+  float* colsum;
+  int ldc, M, N, K;
+};
+hipError_t launch_gemm_f32(const GemmProblem& p0, const GemmProblem* p1, hipStream_t s);
 hipError_t launch_loss_grad(const float* Y, const float* T, int n, int M, int r, float* dY, double* loss,
                             hipStream_t s);
 hipError_t launch_gather_rows(const float* src, const int32_t* idx, int n, int ld, float* dst, hipStream_t s);
@@ -183,16 +191,18 @@ struct cfs_closure {
     float* bufs[2] = {dz0.as<float>(), dz1.as<float>()};
     for (int l = LAYERS - 1; l >= 0; l--) {
       const float* X = l == 0 ? x : h[l - 1].as<float>();
-      // dW = X^T dZ  (K x M), X(m = k_in, k = row), dZ(k = row, n = out)
-      HIP_TRY(launch_gemm_f32(X, 1, K[l], dZ, M[l], 1, g + offW[l], M[l], K[l], M[l], n, nullptr, s), "gemm launch");
-      HIP_TRY(launch_bias_grad(dZ, n, M[l], g + offB[l], s), "bias grad launch");
+      // dW = X^T dZ  (K x M), X(m = k_in, k = row), dZ(k = row, n = out);
+      // db = column sums of dZ
+      const GemmProblem dw{X, 1, K[l], dZ, M[l], 1, g + offW[l], nullptr, g + offB[l], M[l], K[l], M[l], n};
       if (l > 0) {
         // dZ_prev = (dZ W^T) * gelu'(Z_prev): dZ(m = row, k = out), Wq(k = out, n = in) at in * M + out
         float* out = bufs[l & 1];
-        HIP_TRY(launch_gemm_f32(dZ, M[l], 1, wq.as<float>() + offQ[l], 1, M[l], out, K[l], n, K[l], M[l],
-                                z[l - 1].as<float>(), s),
-                "gemm launch");
+        const GemmProblem dx{dZ, M[l], 1, wq.as<float>() + offQ[l], 1, M[l], out, z[l - 1].as<float>(), nullptr,
+                             K[l], n, K[l], M[l]};
+        HIP_TRY(launch_gemm_f32(dw, &dx, s), "gemm launch");
         dZ = out;
+      } else {
+        HIP_TRY(launch_gemm_f32(dw, nullptr, s), "gemm launch");
       }
     }
     return CFS_OK;

@@ -12,8 +12,8 @@
 // layer) predicting the advection increment A(q) - q in scaled units.
 // Weights are BF16 with FP32 master copies for training; the batched forward
 // runs on MFMA (v_mfma_f32_16x16x16_bf16) with each activation split into a
-// BF16 hi + lo pair and FP32 accumulation; the backward pass is FP32 on the
-// vector unit; the optimiser is Adam.  The fused inference step evaluates the
+// BF16 hi + lo pair and FP32 accumulation; the backward products are FP32
+// MFMA (v_mfma_f32_16x16x4_f32); the optimiser is Adam.  The fused inference step evaluates the
 // same network with the BF16 weights and FP32 arithmetic for a few samples
 // per workgroup.
 #include <hip/hip_runtime.h>
@@ -90,52 +90,77 @@ __global__ void __launch_bounds__(256) mlp_fwd_kernel(const float* __restrict__ 
   }
 }
 
-// C[m, n] = sum_k A(m, k) B(k, n), A(m, k) = A[m * sam + k * sak],
-// B(k, n) = B[k * sbk + n * sbn], C row-major with leading dimension ldc.
-// If G is non-null the result is multiplied by gelu'(G[m * ldc + n]).
-// 16 x 16 tiles staged through LDS, one output per thread (FP32).
-__global__ void __launch_bounds__(256) gemm_f32_kernel(const float* __restrict__ A, long sam, long sak,
-                                                       const float* __restrict__ B, long sbk, long sbn,
-                                                       float* __restrict__ C, int ldc, int M, int N, int K,
-                                                       const float* __restrict__ G) {
-  __shared__ float As[16][17], Bs[16][17];
-  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-  const int m = 16 * blockIdx.y + ty, nn = 16 * blockIdx.x + tx;
-  float acc = 0.0f;
-  for (int k0 = 0; k0 < K; k0 += 16) {
-    const int am = 16 * blockIdx.y + ty, ak = k0 + tx;
-    As[ty][tx] = (am < M && ak < K) ? A[am * sam + ak * sak] : 0.0f;
-    const int bk = k0 + ty, bn = 16 * blockIdx.x + tx;
-    Bs[ty][tx] = (bk < K && bn < N) ? B[bk * sbk + bn * sbn] : 0.0f;
-    __syncthreads();
-#pragma unroll
-    for (int kk = 0; kk < 16; kk++) acc += As[ty][kk] * Bs[kk][tx];
-    __syncthreads();
-  }
-  if (m < M && nn < N) {
-    if (G) acc *= gelu_grad(G[(size_t)m * ldc + nn]);
-    C[(size_t)m * ldc + nn] = acc;
-  }
-}
+// One product of a grouped GEMM launch: C[m, n] = sum_k A(m, k) B(k, n) with
+// A(m, k) = A[m * sam + k * sak], B(k, n) = B[k * sbk + n * sbn], C row-major
+// with leading dimension ldc; multiplied by gelu'(G[m * ldc + n]) when G is
+// non-null; colsum[n] = sum_k B(k, n) as well when colsum is non-null.
+struct GemmProblem {
+  const float* A;
+  long sam, sak;
+  const float* B;
+  long sbk, sbn;
+  float* C;
+  const float* G;
+  float* colsum;
+  int ldc, M, N, K;
+};
 
-// db[j] = sum over rows of dZ[row, j]: a workgroup takes 16 columns, its 16
-// row groups sum every 16th row, and the partial sums meet in LDS (fixed
-// order, so the result does not depend on scheduling).
-__global__ void __launch_bounds__(256) bias_grad_kernel(const float* __restrict__ dZ, int n, int M,
-                                                        float* __restrict__ db) {
-  __shared__ float part[16][17];
-  const int c = threadIdx.x & 15, g = threadIdx.x >> 4;
-  const int j = 16 * blockIdx.x + c;
-  float s = 0.0f;
-  if (j < M)
-    for (int row = g; row < n; row += 16) s += dZ[(size_t)row * M + j];
-  part[g][c] = s;
-  __syncthreads();
-  if (g == 0 && j < M) {
-    float t = 0.0f;
+// Two independent products in one launch (workgroups [0, tiles0) take p0,
+// the rest p1).  FP32 on MFMA (v_mfma_f32_16x16x4_f32: FP32 products, FP32
+// accumulation).  A workgroup computes one 16 x 16 tile of C; its 4 waves
+// split K (wave w takes the 16-wide K steps w, w + 4, ...) and their
+// accumulators meet in LDS in a fixed order, so results do not depend on
+// scheduling.  In a K step a lane holds row (or column) l % 16 at
+// k = k0 + 4 (l / 16) + i for the i-th of four MFMAs: every k meets its
+// partner once, and a lane's four k are adjacent.
+__global__ void __launch_bounds__(256) gemm_f32_kernel(GemmProblem p0, GemmProblem p1, int tiles0) {
+  __shared__ f32x4_t part[4][64];
+  __shared__ float cpart[4][64];
+  const bool first = (int)blockIdx.x < tiles0;
+  const GemmProblem& P = first ? p0 : p1;
+  const int t = first ? blockIdx.x : blockIdx.x - tiles0;
+  const int ntn = (P.N + 15) >> 4;
+  const int m0 = 16 * (t / ntn), n0 = 16 * (t % ntn);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int am = m0 + (lane & 15), bn = n0 + (lane & 15), kq = 4 * (lane >> 4);
+  const bool aok = am < P.M, bok = bn < P.N;
+  const bool sums = P.colsum != nullptr && m0 == 0;
+  f32x4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
+  float cs = 0.0f;
+  for (int k0 = 16 * w; k0 < P.K; k0 += 64) {
+    float a[4], b[4];
 #pragma unroll
-    for (int k = 0; k < 16; k++) t += part[k][c];
-    db[j] = t;
+    for (int i = 0; i < 4; i++) {
+      const int k = k0 + kq + i;
+      a[i] = (aok && k < P.K) ? P.A[am * P.sam + k * P.sak] : 0.0f;
+      b[i] = (bok && k < P.K) ? P.B[k * P.sbk + bn * P.sbn] : 0.0f;
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i], b[i], acc, 0, 0, 0);
+    if (sums) cs += (b[0] + b[1]) + (b[2] + b[3]);
+  }
+  part[w][lane] = acc;
+  cpart[w][lane] = cs;
+  __syncthreads();
+  if (w != 0) return;
+  const f32x4_t s0 = part[0][lane], s1 = part[1][lane], s2 = part[2][lane], s3 = part[3][lane];
+  if (!bok) return;
+  // D layout: row 4 (l / 16) + i, column l % 16
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const int row = m0 + kq + i;
+    if (row < P.M) {
+      float v = (s0[i] + s1[i]) + (s2[i] + s3[i]);
+      if (P.G) v *= gelu_grad(P.G[(size_t)row * P.ldc + bn]);
+      P.C[(size_t)row * P.ldc + bn] = v;
+    }
+  }
+  if (sums && lane < 16) {
+    float c = 0.0f;
+#pragma unroll
+    for (int ww = 0; ww < 4; ww++)
+      c += (cpart[ww][lane] + cpart[ww][lane + 16]) + (cpart[ww][lane + 32] + cpart[ww][lane + 48]);
+    P.colsum[bn] = c;
   }
 }
 
@@ -334,17 +359,17 @@ hipError_t launch_mlp_fwd(const float* X, int ldx, int n, const uint16_t* Wp, co
   return hipGetLastError();
 }
 
-hipError_t launch_gemm_f32(const float* A, long sam, long sak, const float* B, long sbk, long sbn, float* C, int ldc,
-                           int M, int N, int K, const float* G, hipStream_t s) {
-  hipLaunchKernelGGL(gemm_f32_kernel, dim3((N + 15) / 16, (M + 15) / 16), dim3(256), 0, s, A, sam, sak, B, sbk, sbn,
-                     C, ldc, M, N, K, G);
+// Grouped launch of the backward products of one layer: the weight gradient
+// (with the bias gradient as its column sums) and, when p1 is non-null, the
+// input gradient.
+hipError_t launch_gemm_f32(const GemmProblem& p0, const GemmProblem* p1, hipStream_t s) {
+  auto tiles = [](const GemmProblem& p) { return ((p.M + 15) / 16) * ((p.N + 15) / 16); };
+  const int t0 = tiles(p0), t1 = p1 ? tiles(*p1) : 0;
+  if (t0 + t1 == 0) return hipSuccess;
+  hipLaunchKernelGGL(gemm_f32_kernel, dim3(t0 + t1), dim3(256), 0, s, p0, p1 ? *p1 : p0, t0);
   return hipGetLastError();
 }
 
-hipError_t launch_bias_grad(const float* dZ, int n, int M, float* db, hipStream_t s) {
-  hipLaunchKernelGGL(bias_grad_kernel, dim3((M + 15) / 16), dim3(256), 0, s, dZ, n, M, db);
-  return hipGetLastError();
-}
 
 hipError_t launch_loss_grad(const float* Y, const float* T, int n, int M, int r, float* dY, double* loss,
                             hipStream_t s) {
