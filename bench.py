@@ -40,6 +40,7 @@ CLOSURE = "closure"  # learned advection closure vs the cubature reduced-advecti
 PLUME = "plume"  # the whole plume timestep on the device (stepPlume), two bases, HIP-graph replay
 PIPELINE = "pipeline"  # full-rank plume -> SVD bases -> reduced operators -> cubature -> compressed step, accuracy
 PLUME_CLOSURE = "plume-closure"  # the learned closure inside the real reduced plume, against its cubature
+OBSTACLE = "obstacle"  # the moving-obstacle chain: full-rank box run -> bases with U.iop -> reduced obstacle step
 ACCURACY = "accuracy"  # compressed vs uncompressed reduced plume, per-step q error, 48x64x48 r=52
 
 
@@ -358,6 +359,84 @@ def run_plume(args):
     print(json.dumps(result), flush=True)
 
 
+def run_obstacle(args):
+    """The moving-obstacle chain on one MI355X: the full-rank solver stepped
+    with the spinning, sliding box (stepWithMovingObstacle, the box of
+    buildProducts.cpp: centre 0.5, lengths 0.4 x 0.05 x 0.15, periods 4 and
+    10), SVD bases including U.iop, the IOP products, the staged cubature and
+    the compressed bases, then the reduced obstacle step (stepMovingObstacle,
+    the box's R q + n evaluated from its cells) with the compressed and the
+    uncompressed bases against the full-rank run; one reduced obstacle frame
+    timed (compressed bases, cubature advection)."""
+    from cfs_plume_amd import accuracy as A
+    from cfs_plume_amd import obstacle as OBX
+    from cfs_plume_amd import pipeline as PP
+    from cfs_plume_amd.cubature import Cubature
+    from cfs_plume_amd.hip import Stream
+    from cfs_plume_amd.plume import ADVECT_CUBATURE, PlumeSolver, default_params
+
+    grid = tuple(int(g) for g in args.pipeline_grid.split("x"))
+    steps = args.pipeline_steps
+    t0 = time.perf_counter()
+    data = PP.build_plume_data(grid, steps, cubature_snapshots=args.pipeline_cubature_snapshots,
+                               candidates=args.pipeline_candidates, tolerance=args.pipeline_tolerance,
+                               threads=args.threads, obstacle=OBX.Box(), log=lambda m: log(f"[bench] {m}"))
+    t_build = time.perf_counter() - t0
+    ranks = {k: b.r for k, b in data.bases.items()}
+    log(f"[bench] obstacle chain built in {t_build:.1f}s: ranks {ranks}, cubature {len(data.cubature.cells)} cells, "
+        f"timings {data.timings}")
+    t0 = time.perf_counter()
+    acc = A.compare_plume(data, steps)
+    t_runs = time.perf_counter() - t0
+    e = acc["q_rel_l2"]
+    last = {name: round(float(acc[name]["velocity_vs_full_rank"][-1]), 8) for name in
+            ("compressed", "uncompressed", "uncompressed_exact") if name in acc}
+    log(f"[bench] obstacle: compressed vs uncompressed q max {e.max():.3e}; velocity vs full rank at the last step "
+        f"{last} ({t_runs:.1f}s)")
+    # one reduced obstacle frame (compressed bases, cubature), the box held where it starts
+    ops = data.operators
+    dims = data.peeled
+    comp = PP.compressed_bases(data)
+    prm = default_params(data.full)
+    prm.vorticity_eps = 0.0
+    zeros = np.zeros((data.bases["final"].r, ops["damping.peeled"].shape[0]))
+    solver = PlumeSolver(comp["preadvect"], comp["final"], ops["damping.peeled"], zeros, zeros, prm)
+    cub = Cubature.from_rows(data.cubature.cells, data.cubature.weights, dims, data.bases["prediffuse"].numpy())
+    solver.set_advection(ADVECT_CUBATURE, cubature=cub)
+    solver.set_obstacle(data.bases["iop"].U, data.bases["preproject"].U, ops["U'U.iop.subspace"],
+                        ops["projected.ptof"], ops["inverseProduct"])
+    box = OBX.Box()
+    stream = Stream()
+    vel, dens, heat, q = solver.fields()
+    for _ in range(10):
+        solver.frame_obstacle(box, vel, dens, heat, q, stream)
+    ms, per = time_launches(lambda: solver.step_obstacle(box, vel, dens, heat, q, stream), args.steps, args.warmup,
+                            stream)
+    full_ms = 1e3 * data.timings["full_rank_s"] / steps
+    result = {
+        "metric": f"reduced moving-obstacle frame time (stam grid {grid[0]}x{grid[1]}x{grid[2]}, compressed bases, "
+                  f"cubature advection)",
+        "value": round(ms, 5), "unit": "ms", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms, 5), "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp64 full-rank and bases, fp32 reduced fields, fp64 q",
+        "data": "simulated (full-rank moving-obstacle run on the device)",
+        "config": {"workload": f"stepWithMovingObstacle x {steps} -> svdOutOfCoreMultiple (iop) -> "
+                               f"buildProducts (iop) -> cubature -> compressMatrices -> stepMovingObstacle",
+                   "grid": list(grid), "ranks": ranks, "parallelism": "single GPU",
+                   "box": {"center": [0.5, 0.5, 0.5], "lengths": [0.4, 0.05, 0.15], "period": 4.0,
+                           "translation_period": 10.0}},
+        "accuracy": {"q_rel_l2_compressed_vs_uncompressed_max": round(float(e.max()), 6),
+                     "q_rel_l2_at": {str(s): round(float(e[s - 1]), 6) for s in (1, 10, 50, 100) if s <= steps},
+                     "velocity_vs_full_rank_last": last},
+        "full_rank_ms_per_step": round(full_ms, 3), "speedup_over_full_rank": round(full_ms / ms, 1),
+        "median_ms": round(float(np.median(per)), 5),
+        "timings_s": {k: round(v, 2) for k, v in data.timings.items()} | {"reduced_runs": round(t_runs, 2)},
+    }
+    solver.close()
+    cub.close()
+    print(json.dumps(result), flush=True)
+
+
 def run_pipeline(args):
     """The reference's whole offline chain on one MI355X (cfg stam.64: a
     48x64x48 Stam plume, 100 full-rank steps, SVD bases with the 1e-9 keep
@@ -596,7 +675,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default=DEFAULT, choices=sorted(WORKLOADS) + [BATCH, CLOSURE, ACCURACY, PLUME, PIPELINE, PLUME_CLOSURE])
+    ap.add_argument("--workload", default=DEFAULT, choices=sorted(WORKLOADS) + [BATCH, CLOSURE, ACCURACY, PLUME, PIPELINE, PLUME_CLOSURE, OBSTACLE])
     ap.add_argument("--pipeline-grid", default="48x64x48", help="full grid of the pipeline workload")
     ap.add_argument("--pipeline-steps", type=int, default=100, help="full-rank steps and reduced frames")
     ap.add_argument("--pipeline-cubature-snapshots", type=int, default=100,
@@ -651,6 +730,9 @@ def main():
         return
     if args.workload == PLUME:
         run_plume(args)
+        return
+    if args.workload == OBSTACLE:
+        run_obstacle(args)
         return
     if args.workload == PIPELINE:
         run_pipeline(args)

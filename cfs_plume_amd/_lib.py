@@ -92,6 +92,14 @@ SIGNATURES = {
     "cfs_fluid_step": (_i, [_vp, _vp, _vp]),
     "cfs_fluid_last_solve": (_i, [_vp, ctypes.POINTER(_i), ctypes.POINTER(ctypes.c_double)]),
     "cfs_fluid_destroy": (_i, [_vp]),
+    "cfs_box_init": (_i, [_vp, _vp, ctypes.c_double, ctypes.c_double, ctypes.c_double, _vp]),
+    "cfs_box_advance": (_i, [_vp]),
+    "cfs_fluid_set_initial_velocity": (_i, [_vp, _vp, _vp]),
+    "cfs_fluid_step_obstacle": (_i, [_vp, _vp, _vp, _vp, _vp]),
+    "cfs_reduced_iop_scratch_bytes": (ctypes.c_int64, [_i, _i, _i]),
+    "cfs_reduced_iop": (_i, [_vp, _i, _vp, _i, _vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp]),
+    "cfs_plume_set_obstacle": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp]),
+    "cfs_plume_step_obstacle": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "cfs_gemm_tn": (_i, [_vp, ctypes.c_int64, _i, ctypes.c_int64, _vp, _i, ctypes.c_int64, _vp, _vp]),
     "cfs_gemm_nn": (_i, [_vp, ctypes.c_int64, _i, ctypes.c_int64, _vp, _i, _vp, ctypes.c_int64, _vp]),
     "cfs_apply_operator": (_i, [_i, _i, _i, _i, _vp, ctypes.c_double, ctypes.c_double, _vp, ctypes.c_int64, _i, _vp,
@@ -129,6 +137,20 @@ class FluidParams(ctypes.Structure):
         ("solver_eps", ctypes.c_double),
         ("iterations", ctypes.c_int),
         ("boundaries", ctypes.c_int * 6),
+    ]
+
+
+class Box(ctypes.Structure):
+    _fields_ = [
+        ("center", ctypes.c_double * 3),
+        ("original_center", ctypes.c_double * 3),
+        ("half_lengths", ctypes.c_double * 3),
+        ("period", ctypes.c_double),
+        ("translation_period", ctypes.c_double),
+        ("theta", ctypes.c_double),
+        ("phi", ctypes.c_double),
+        ("time", ctypes.c_double),
+        ("dt", ctypes.c_double),
     ]
 
 

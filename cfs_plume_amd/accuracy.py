@@ -114,7 +114,10 @@ def compare_plume(data, steps: int, stream: Optional[Stream] = None, out_prefix:
     compareSubspace per step, and the reduced velocity of each run against
     the full-rank run's final velocity of the same step (diffGroundTruth).
 
-    `data` is a pipeline.PlumeData.  Returns per-step arrays."""
+    `data` is a pipeline.PlumeData.  With a moving obstacle (data.box), every
+    run steps with stepMovingObstacle from the box's starting position, the
+    IOP products of `data.operators` and U_iop / U_preproject dense on the
+    device.  Returns per-step arrays."""
     from .cubature import Cubature
     from .hip import DeviceArray
     from .pipeline import compressed_bases
@@ -123,6 +126,11 @@ def compare_plume(data, steps: int, stream: Optional[Stream] = None, out_prefix:
     dims = data.peeled
     ops = data.operators
     D, P, I = ops["damping.peeled"], ops["projected.ptof"], ops["inverseProduct"]
+    box0 = getattr(data, "box", None)
+    if box0 is not None:
+        # the plain projection matrices are not used by the obstacle step
+        P_iop, I_iop = P, I
+        P = I = np.zeros((data.bases["final"].r, D.shape[0]))
     prm = default_params(data.full)
     prm.vorticity_eps = 0.0  # the drivers' cfg value ("vorticity", default 0)
     cub = Cubature.from_rows(data.cubature.cells, data.cubature.weights, dims, data.bases["prediffuse"].numpy())
@@ -145,12 +153,20 @@ def compare_plume(data, steps: int, stream: Optional[Stream] = None, out_prefix:
             solver.set_advection(ADVECT_EXACT)
         else:
             solver.set_advection(ADVECT_CUBATURE, cubature=cub)
+        box = None
+        if box0 is not None:
+            solver.set_obstacle(data.bases["iop"].U, data.bases["preproject"].U, ops["U'U.iop.subspace"], P_iop,
+                                I_iop)
+            box = box0.copy()
         vel, dens, heat, q = solver.fields()
         qs = np.zeros((steps, r_fin))
         vdiff = np.zeros(steps)
         X, Y, Z = data.full
         for t in range(steps):
-            solver.frame(vel, dens, heat, q, stream)
+            if box is not None:
+                solver.frame_obstacle(box, vel, dens, heat, q, stream)
+            else:
+                solver.frame(vel, dens, heat, q, stream)
             qs[t] = q.numpy()[:r_fin]
             v = vel.numpy().reshape(Z, Y, X, 3)[1:-1, 1:-1, 1:-1].reshape(-1).astype(np.float64)
             if t < truth.shape[0]:

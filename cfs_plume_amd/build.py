@@ -20,7 +20,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libcfs_plume.so")
 ARCH = "gfx950"
 
-HIP_SOURCES = ["step_kernels.hip", "decode_kernels.hip", "closure_kernels.hip", "noalloc.hip", "cubature.hip", "plume.hip", "fluid.hip", "linalg.hip"]
+HIP_SOURCES = ["step_kernels.hip", "decode_kernels.hip", "closure_kernels.hip", "noalloc.hip", "cubature.hip", "plume.hip", "fluid.hip", "linalg.hip", "obstacle.hip"]
 HIP_HOST_SOURCES = ["basis.cpp", "batch.cpp", "closure.cpp"]
 CXX_SOURCES = ["encoder.cpp", "nnls.cpp"]
 

@@ -38,6 +38,7 @@
 #include "../../include/cfs_plume.h"
 #include "basis_internal.h"
 #include "grid_ops.h"
+#include "obstacle_internal.h"
 
 using namespace cfs;  // the shared grid stencils
 
@@ -546,8 +547,15 @@ int cfs_fluid_add_smoke(cfs_fluid* f, void* stream) {
   return CFS_OK;
 }
 
-int cfs_fluid_step(cfs_fluid* f, const cfs_fluid_snapshots* snap, void* stream) {
-  if (!f) return cfs::fail(CFS_ERR_ARG, "null fluid");
+}  // extern "C"
+
+namespace {
+
+// FLUID_3D_MIC::step, or with a box stepWithMovingObstacle (:236-310): no
+// buoyancy, and the box's cells stomped between the damping and the
+// projection (the post-stomp velocity is the IOP snapshot).
+int fluid_step(cfs_fluid* f, const cfs_fluid_snapshots* snap, const cfs_box* box, double* iop_snapshot,
+               void* stream) {
   cfs::DeviceGuard g(f->device);
   const hipStream_t s = (hipStream_t)stream;
   const int X = f->X, Y = f->Y, Z = f->Z;
@@ -563,7 +571,8 @@ int cfs_fluid_step(cfs_fluid* f, const cfs_fluid_snapshots* snap, void* stream) 
   };
   // forces: velocity and density borders zeroed, buoyancy (the force is
   // added to every cell), vorticity confinement
-  hipLaunchKernelGGL(buoyancy_kernel<double>, grid, block, 0, s, f->vel, f->heat, X, Y, Z, dt, P.buoyancy);
+  hipLaunchKernelGGL(buoyancy_kernel<double>, grid, block, 0, s, f->vel, f->heat, X, Y, Z, dt,
+                     box ? 0.0 : P.buoyancy);
   hipLaunchKernelGGL(zero_border2_kernel<double>, grid, block, 0, s, f->dens, (double*)nullptr, X, Y, Z);
   if (P.vorticity_eps != 0.0) {
     hipLaunchKernelGGL(curl_kernel<double>, grid, block, 0, s, f->vel, f->w, f->wmag, X, Y, Z, inv_dx);
@@ -592,8 +601,12 @@ int cfs_fluid_step(cfs_fluid* f, const cfs_fluid_snapshots* snap, void* stream) 
   hipLaunchKernelGGL(damping_kernel, grid, block, 0, s, f->vel, f->vel_old, X, Y, Z, P.damping_w);
   hipLaunchKernelGGL(copy_interior_kernel, grid, block, 0, s, f->vel_old, f->vel, X, Y, Z);
   FL_HIP(hipGetLastError(), "damping launch");
-  // project
   if (snap) peel(f->vel, snap->preproject, 3);
+  if (box) {
+    FL_HIP(cfs::launch_box_stomp(f->vel, cfs::box_state(*box, X, Y, Z), s), "box stomp launch");
+    peel(f->vel, iop_snapshot, 3);
+  }
+  // project
   hipLaunchKernelGGL(velocity_border_kernel<double>, grid, block, 0, s, f->vel, X, Y, Z, open_x, open_y, open_z, 2);
   hipLaunchKernelGGL(divergence_kernel, grid, block, 0, s, f->vel, f->obst, f->div, X, Y, Z, P.dx);
   // the pressure's interior is reset to 0 and copyBorderAll copies zeros
@@ -610,6 +623,29 @@ int cfs_fluid_step(cfs_fluid* f, const cfs_fluid_snapshots* snap, void* stream) 
     peel(f->div, snap->divergence, 1);
     FL_HIP(hipGetLastError(), "snapshot launch");
   }
+  return CFS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cfs_fluid_step(cfs_fluid* f, const cfs_fluid_snapshots* snap, void* stream) {
+  if (!f) return cfs::fail(CFS_ERR_ARG, "null fluid");
+  return fluid_step(f, snap, nullptr, nullptr, stream);
+}
+
+int cfs_fluid_step_obstacle(cfs_fluid* f, const cfs_box* box, const cfs_fluid_snapshots* snap, double* iop_snapshot,
+                            void* stream) {
+  if (!f || !box) return cfs::fail(CFS_ERR_ARG, "null argument");
+  return fluid_step(f, snap, box, iop_snapshot, stream);
+}
+
+int cfs_fluid_set_initial_velocity(cfs_fluid* f, const cfs_box* box, void* stream) {
+  if (!f || !box) return cfs::fail(CFS_ERR_ARG, "null argument");
+  cfs::DeviceGuard g(f->device);
+  FL_HIP(cfs::launch_box_stomp(f->vel, cfs::box_state(*box, f->X, f->Y, f->Z), (hipStream_t)stream),
+         "box stomp launch");
   return CFS_OK;
 }
 

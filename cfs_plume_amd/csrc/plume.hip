@@ -34,6 +34,7 @@
 #include "../../include/cfs_plume.h"
 #include "basis_internal.h"
 #include "grid_ops.h"
+#include "obstacle_internal.h"
 
 using namespace cfs;  // the shared grid stencils
 
@@ -81,8 +82,9 @@ __global__ void copy_q_kernel(const double* __restrict__ a, double* __restrict__
   if ((int)threadIdx.x < r) b[threadIdx.x] = a[threadIdx.x];
 }
 
-// q <- D q (r_proj x r_adv), then q <- P q + I q (r_fin x r_proj), FP64,
-// every dimension <= 256, one workgroup.  q_in and q_out may alias.
+// q <- D q (r_proj x r_adv; D null: the identity, r_proj = r_adv), then
+// q <- P q + I q (r_fin x r_proj), FP64, every dimension <= 256, one
+// workgroup.  q_in and q_out may alias.
 __global__ void __launch_bounds__(MAX_R) reduced_ops_kernel(const double* __restrict__ D, const double* __restrict__ P,
                                                             const double* __restrict__ I, const double* q_in,
                                                             double* q_out, int r_adv, int r_proj, int r_fin) {
@@ -92,7 +94,11 @@ __global__ void __launch_bounds__(MAX_R) reduced_ops_kernel(const double* __rest
   __syncthreads();
   if (j < r_proj) {
     double s = 0.0;
-    for (int k = 0; k < r_adv; k++) s += D[(size_t)j * r_adv + k] * a[k];
+    if (D) {
+      for (int k = 0; k < r_adv; k++) s += D[(size_t)j * r_adv + k] * a[k];
+    } else {
+      s = a[j];
+    }
     t[j] = s;
   }
   __syncthreads();
@@ -104,6 +110,16 @@ __global__ void __launch_bounds__(MAX_R) reduced_ops_kernel(const double* __rest
     }
     q_out[j] = p + c;
   }
+}
+
+// y = A x (rows x cols, row-major), one workgroup, dimensions <= 256
+__global__ void __launch_bounds__(MAX_R) matvec_kernel(const double* __restrict__ A, const double* __restrict__ x,
+                                                       double* __restrict__ y, int rows, int cols) {
+  const int j = threadIdx.x;
+  if (j >= rows) return;
+  double s = 0.0;
+  for (int k = 0; k < cols; k++) s += A[(size_t)j * cols + k] * x[k];
+  y[j] = s;
 }
 
 }  // namespace
@@ -127,6 +143,14 @@ struct cfs_plume_solver {
   const double* U_fin_dense = nullptr;
   const double* U_adv_dense = nullptr;  // dense exact advection: project onto this after-basis
   double* d_col = nullptr;  // an FP64 peeled vector field
+  // moving obstacle (cfs_plume_set_obstacle): dense U_iop and U_preproject
+  // (device, owned by the caller), U_iop^T U_preproject, and the projection
+  // matrices of the IOP basis (r_fin x r_iop)
+  int r_iop = 0;
+  const double* U_iop = nullptr;
+  const double* U_proj = nullptr;
+  double *d_M = nullptr, *d_Piop = nullptr, *d_Iiop = nullptr, *d_t = nullptr, *d_u = nullptr;
+  void* d_iop_scratch = nullptr;
   float *d_w = nullptr, *d_wmag = nullptr, *d_dold = nullptr, *d_hold = nullptr, *d_fa = nullptr, *d_fb = nullptr;
   std::vector<void*> allocs;
   ~cfs_plume_solver() {
@@ -336,8 +360,39 @@ int cfs_plume_add_smoke(cfs_plume_solver* p, float* density, float* heat, void* 
   return CFS_OK;
 }
 
-int cfs_plume_step(cfs_plume_solver* p, float* velocity, float* density, float* heat, double* q, void* stream) {
-  if (!p || !velocity || !density || !heat || !q) return cfs::fail(CFS_ERR_ARG, "null argument");
+int cfs_plume_set_obstacle(cfs_plume_solver* p, int r_iop, const double* U_iop, const double* U_proj,
+                           const double* iop_T_proj, const double* ptof_iop, const double* inverse_iop) {
+  if (!p || !U_iop || !U_proj || !iop_T_proj || !ptof_iop || !inverse_iop)
+    return cfs::fail(CFS_ERR_ARG, "null argument");
+  if (r_iop < 1 || r_iop > MAX_R) return cfs::fail(CFS_ERR_ARG, "r_iop must lie in [1, 256]");
+  if (p->r_iop) return cfs::fail(CFS_ERR_ARG, "the plume already has an obstacle");
+  cfs::DeviceGuard g(p->device);
+  int st;
+  if ((st = dev_alloc(p, &p->d_M, (size_t)r_iop * p->r_proj, iop_T_proj))) return st;
+  if ((st = dev_alloc(p, &p->d_Piop, (size_t)p->r_fin * r_iop, ptof_iop))) return st;
+  if ((st = dev_alloc(p, &p->d_Iiop, (size_t)p->r_fin * r_iop, inverse_iop))) return st;
+  if ((st = dev_alloc<double>(p, &p->d_t, (size_t)MAX_R, nullptr))) return st;
+  if ((st = dev_alloc<double>(p, &p->d_u, (size_t)MAX_R, nullptr))) return st;
+  const int64_t bytes = cfs_reduced_iop_scratch_bytes(p->X - 2, p->Y - 2, p->Z - 2);
+  uint8_t* scratch = nullptr;
+  if ((st = dev_alloc<uint8_t>(p, &scratch, (size_t)bytes, nullptr))) return st;
+  p->d_iop_scratch = scratch;
+  p->U_iop = U_iop;
+  p->U_proj = U_proj;
+  p->r_iop = r_iop;
+  return CFS_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// stepPlume, or with a box stepMovingObstacle (SUBSPACE_FLUID_3D_COMPRESSED_
+// EIGEN.cpp:508-566): no buoyancy, and between the diffusion and the pressure
+// projection q = R q + n (reducedSetMovingBox :779-790), the projection then
+// with the IOP basis' matrices.
+int plume_step(cfs_plume_solver* p, const cfs_box* box, float* velocity, float* density, float* heat, double* q,
+               void* stream) {
   if (p->mode == CFS_PLUME_ADVECT_EXACT && p->r_adv != p->r_pre && !p->U_adv_dense)
     return cfs::fail(CFS_ERR_ARG, "no advection set: the exact advection needs r_adv equal to U_preadvect's r");
   cfs::DeviceGuard g(p->device);
@@ -348,7 +403,8 @@ int cfs_plume_step(cfs_plume_solver* p, float* velocity, float* density, float* 
   const float dt0 = (float)(p->prm.dt / p->prm.dx);
   int st;
   // forces
-  hipLaunchKernelGGL(buoyancy_kernel<float>, grid, block, 0, s, velocity, heat, X, Y, Z, dt, (float)p->prm.buoyancy);
+  hipLaunchKernelGGL(buoyancy_kernel<float>, grid, block, 0, s, velocity, heat, X, Y, Z, dt,
+                     box ? 0.0f : (float)p->prm.buoyancy);
   PL_HIP(hipGetLastError(), "buoyancy launch");
   if (p->prm.vorticity_eps > 0.0) {
     hipLaunchKernelGGL(curl_kernel<float>, grid, block, 0, s, velocity, p->d_w, p->d_wmag, X, Y, Z, inv_dx);
@@ -398,12 +454,42 @@ int cfs_plume_step(cfs_plume_solver* p, float* velocity, float* density, float* 
     }
   }
   if (st) return st;
-  // diffusion and pressure projection
-  hipLaunchKernelGGL(reduced_ops_kernel, dim3(1), dim3(MAX_R), 0, s, p->d_D, p->d_P, p->d_I, p->d_q, q, p->r_adv,
-                     p->r_proj, p->r_fin);
+  if (box) {
+    // diffusion, the box, the pressure projection with the IOP basis
+    hipLaunchKernelGGL(matvec_kernel, dim3(1), dim3(MAX_R), 0, s, p->d_D, p->d_q, p->d_t, p->r_proj, p->r_adv);
+    PL_HIP(hipGetLastError(), "diffusion launch");
+    const int64_t n = (int64_t)(X - 2) * (Y - 2) * (Z - 2);
+    double* w = static_cast<double*>(p->d_iop_scratch);
+    uint8_t* flags = reinterpret_cast<uint8_t*>(w + 3 * n);
+    PL_HIP(cfs::launch_reduced_iop(p->U_iop, p->r_iop, p->U_proj, p->r_proj, p->d_M, cfs::box_state(*box, X, Y, Z),
+                                   p->d_t, p->d_u, w, flags, s),
+           "reduced IOP launch");
+    hipLaunchKernelGGL(reduced_ops_kernel, dim3(1), dim3(MAX_R), 0, s, (const double*)nullptr, p->d_Piop, p->d_Iiop,
+                       p->d_u, q, p->r_iop, p->r_iop, p->r_fin);
+  } else {
+    // diffusion and pressure projection
+    hipLaunchKernelGGL(reduced_ops_kernel, dim3(1), dim3(MAX_R), 0, s, p->d_D, p->d_P, p->d_I, p->d_q, q, p->r_adv,
+                       p->r_proj, p->r_fin);
+  }
   PL_HIP(hipGetLastError(), "reduced operators launch");
   // unprojection
   return unproject_with(p, 1, q, velocity, stream);
+}
+
+}  // namespace
+
+extern "C" {
+
+int cfs_plume_step(cfs_plume_solver* p, float* velocity, float* density, float* heat, double* q, void* stream) {
+  if (!p || !velocity || !density || !heat || !q) return cfs::fail(CFS_ERR_ARG, "null argument");
+  return plume_step(p, nullptr, velocity, density, heat, q, stream);
+}
+
+int cfs_plume_step_obstacle(cfs_plume_solver* p, const cfs_box* box, float* velocity, float* density, float* heat,
+                            double* q, void* stream) {
+  if (!p || !box || !velocity || !density || !heat || !q) return cfs::fail(CFS_ERR_ARG, "null argument");
+  if (!p->r_iop) return cfs::fail(CFS_ERR_ARG, "no obstacle set (cfs_plume_set_obstacle)");
+  return plume_step(p, box, velocity, density, heat, q, stream);
 }
 
 int cfs_plume_set_params(cfs_plume_solver* p, const cfs_plume_params* prm) {

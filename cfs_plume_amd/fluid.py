@@ -7,6 +7,9 @@ Reference (aadjones/CFS_PLUME) -> here:
   step (FLUID_3D_MIC.cpp:179-233)                      FluidSolver.step
   appendStreams (FLUID_3D_MIC.cpp:1329-1480)           Snapshots (the six streams, on the device)
   projects/fluid3D runEverytime (fluid3D.cpp:357-385)  FluidSolver.run
+  setInitialVelocity (FLUID_3D_MIC.cpp:744-763)        FluidSolver.set_initial_velocity
+  stepWithMovingObstacle (FLUID_3D_MIC.cpp:236-310)    FluidSolver.step_obstacle, run_obstacle
+                                                       (the IOP stream of appendStreams :1431-1439)
 
 The fields are FP64 (the reference's Real) and stay on the device; a run
 fills device snapshot matrices, one column per step, that the basis builder
@@ -25,6 +28,7 @@ from .basis import _sptr
 from .hip import DeviceArray, Stream
 
 STREAMS = ("preadvect", "prediffuse", "preproject", "final_velocity", "pressure", "divergence")
+IOP_STREAM = "iop"  # the post-stomp velocity of the obstacle step
 
 
 def default_params(full_dims, vorticity: float = 0.0) -> _lib.FluidParams:
@@ -87,6 +91,21 @@ class FluidSolver:
         _lib.check(_lib.lib().cfs_fluid_step(self._h, ctypes.byref(snap) if snap is not None else None,
                                              _sptr(stream)), "FLUID_3D_MIC::step")
 
+    def set_initial_velocity(self, box, stream: Optional[Stream] = None):
+        _lib.check(_lib.lib().cfs_fluid_set_initial_velocity(self._h, box.ref, _sptr(stream)), "setInitialVelocity")
+
+    def step_obstacle(self, box, snapshots: Optional[Snapshots] = None, t: int = 0,
+                      stream: Optional[Stream] = None):
+        """stepWithMovingObstacle with the box at its current position."""
+        snap, iop = None, None
+        if snapshots is not None:
+            snap = _lib.FluidSnapshots(*[snapshots.column_ptr(name, t) for name in STREAMS])
+            if IOP_STREAM in snapshots.arrays:
+                iop = snapshots.column_ptr(IOP_STREAM, t)
+        _lib.check(_lib.lib().cfs_fluid_step_obstacle(self._h, box.ref, ctypes.byref(snap) if snap is not None
+                                                      else None, iop, _sptr(stream)),
+                   "FLUID_3D_MIC::stepWithMovingObstacle")
+
     def last_solve(self):
         """(CG iterations, residual maximum) of the last pressure solve."""
         it = ctypes.c_int(0)
@@ -94,10 +113,11 @@ class FluidSolver:
         _lib.check(_lib.lib().cfs_fluid_last_solve(self._h, ctypes.byref(it), ctypes.byref(mx)), "last solve")
         return it.value, mx.value
 
-    def new_snapshots(self, steps: int) -> Snapshots:
+    def new_snapshots(self, steps: int, iop: bool = False) -> Snapshots:
         np_ = self.peeled_cells
+        names = STREAMS + ((IOP_STREAM,) if iop else ())
         arrays = {name: DeviceArray.zeros((steps, (1 if name in ("pressure", "divergence") else 3) * np_),
-                                          np.float64) for name in STREAMS}
+                                          np.float64) for name in names}
         return Snapshots(arrays, steps, np_)
 
     def run(self, steps: int, stream: Optional[Stream] = None, log=None) -> Snapshots:
@@ -110,6 +130,26 @@ class FluidSolver:
             if log is not None and (t % 10 == 0 or t == steps - 1):
                 it, mx = self.last_solve()
                 log(f"full-rank step {t + 1}/{steps}: {it} CG iterations, residual max {mx:.2e}")
+        return snaps
+
+    def run_obstacle(self, steps: int, box, smoke: bool = True, initial_velocity: bool = False,
+                     stream: Optional[Stream] = None, log=None) -> Snapshots:
+        """A moving-obstacle run: optionally the box's velocity set into its
+        cells first (setInitialVelocity), then per step (passive) smoke, the
+        obstacle step with the box where it stands, the seven streams, and the
+        box advanced (translate_center, spin, update_time) -- the schedule for
+        which buildObstacleMatrices writes its per-step pairs."""
+        snaps = self.new_snapshots(steps, iop=True)
+        if initial_velocity:
+            self.set_initial_velocity(box, stream)
+        for t in range(steps):
+            if smoke:
+                self.add_smoke(stream)
+            self.step_obstacle(box, snaps, t, stream)
+            box.advance()
+            if log is not None and (t % 10 == 0 or t == steps - 1):
+                it, mx = self.last_solve()
+                log(f"full-rank obstacle step {t + 1}/{steps}: {it} CG iterations, residual max {mx:.2e}")
         return snaps
 
     def close(self):

@@ -130,19 +130,28 @@ def svd_basis(snaps: DeviceArray, threshold: float = 1e-9, norm_floor: float = 1
 
 
 def build_products(full, boundaries, dx: float, bases: dict, w: float = 0.9,
-                   stream: Optional[Stream] = None) -> dict:
+                   stream: Optional[Stream] = None, iop: bool = False) -> dict:
     """buildOutOfCoreMatrices: the reduced operators from the bases
     (pressure, preproject, prediffuse, final) and the sparse full-grid
-    operators, FP64 on the host (names as the reference's files)."""
+    operators, FP64 on the host (names as the reference's files).
+
+    iop: buildOutOfCoreMatricesIOP (SUBSPACE_FLUID_3D_EIGEN.cpp:1494-1659):
+    the pressure projection acts on the post-stomp velocity, so vtod and ptof
+    take U_iop (bases["iop"]) in place of U_preproject, and U_iop^T
+    U_preproject ("U'U.iop.subspace", :1635-1637) is added."""
     Up, Uproj, Udiff, Ufin = (bases[k].U for k in ("pressure", "preproject", "prediffuse", "final"))
+    Uvel = bases["iop"].U if iop else Uproj
     A = gemm_tn(Up, apply_operator(OP_POISSON, full, boundaries, dx, w, Up, stream), stream)
-    vtod = gemm_tn(Up, apply_operator(OP_VEL_TO_DIV, full, boundaries, dx, w, Uproj, stream), stream)
+    vtod = gemm_tn(Up, apply_operator(OP_VEL_TO_DIV, full, boundaries, dx, w, Uvel, stream), stream)
     damping = gemm_tn(Uproj, apply_operator(OP_DAMPING, full, boundaries, dx, w, Udiff, stream), stream)
-    ptof = gemm_tn(Ufin, Uproj, stream)
+    ptof = gemm_tn(Ufin, Uvel, stream)
     ptov = gemm_tn(Ufin, apply_operator(OP_PRESSURE_TO_VEL, full, boundaries, dx, w, Up, stream), stream)
     inverse = ptov @ np.linalg.inv(A) @ vtod
-    return {"projected.A": A, "projected.vtod": vtod, "damping.peeled": damping, "projected.ptof": ptof,
-            "projected.ptov": ptov, "inverseProduct": inverse}
+    out = {"projected.A": A, "projected.vtod": vtod, "damping.peeled": damping, "projected.ptof": ptof,
+           "projected.ptov": ptov, "inverseProduct": inverse}
+    if iop:
+        out["U'U.iop.subspace"] = gemm_tn(bases["iop"].U, Uproj, stream)
+    return out
 
 
 def training_pairs(snaps: Snapshots, bases: dict, count: int) -> tuple:
@@ -174,6 +183,7 @@ class PlumeData:
     cubature: Optional[CubatureResult] = None
     compressed: dict = field(default_factory=dict)
     timings: dict = field(default_factory=dict)
+    box: Optional[object] = None  # the obstacle's box at the run's start (moving-obstacle data)
 
     @property
     def peeled(self):
@@ -183,23 +193,31 @@ class PlumeData:
 def build_plume_data(full=(48, 64, 48), steps: int = 100, threshold: float = 1e-9, cubature_snapshots: int = 20,
                      candidates: int = 1000, tolerance: float = 0.01, passes: int = 6, percent: float = 0.9995,
                      n_bits: int = 32, max_iterations: int = 50, threads: int = 16, vorticity: float = 0.0,
-                     log: Optional[Callable[[str], None]] = None) -> PlumeData:
+                     obstacle=None, log: Optional[Callable[[str], None]] = None) -> PlumeData:
     """The reference's offline chain for one plume (cfg stam.64: 48 x 64 x 48,
     100 snapshots, percent 0.9995, nBits 32, maxIterations 50; cubature from
-    the first 20 snapshot pairs, 1000 candidates per pass, error 0.01)."""
+    the first 20 snapshot pairs, 1000 candidates per pass, error 0.01).
+
+    obstacle: an obstacle.Box -- the moving-obstacle chain instead: the
+    full-rank run steps with the box (stepWithMovingObstacle, no buoyancy,
+    passive smoke), the IOP stream gets its basis U.iop, and the products
+    are the IOP ones (build_products(iop=True)); the box is kept at its
+    starting position in PlumeData.box."""
     import time
     say = log or (lambda m: None)
     t = {}
     t0 = time.perf_counter()
     fluid = FluidSolver(full, vorticity=vorticity)
-    snaps = fluid.run(steps, log=say)
+    start_box = obstacle.copy() if obstacle is not None else None
+    snaps = fluid.run_obstacle(steps, obstacle.copy(), log=say) if obstacle is not None else fluid.run(steps, log=say)
     prm = fluid.params
     boundaries, dx = list(prm.boundaries), prm.dx
     dt0 = prm.dt / prm.dx
     t["full_rank_s"] = time.perf_counter() - t0
     t0 = time.perf_counter()
     bases = {}
-    for stream_name, name in STREAM_BASES.items():
+    stream_bases = dict(STREAM_BASES, **({"iop": "iop"} if obstacle is not None else {}))
+    for stream_name, name in stream_bases.items():
         bases[name] = svd_basis(snaps.arrays[stream_name], threshold)
         say(f"U.{name}: {bases[name].r} modes of {bases[name].snapshots} snapshots")
     # the pressure stream holds the pressure and the divergence of every step
@@ -210,9 +228,9 @@ def build_plume_data(full=(48, 64, 48), steps: int = 100, threshold: float = 1e-
     say(f"U.pressure: {bases['pressure'].r} modes of {bases['pressure'].snapshots} snapshots")
     t["svd_s"] = time.perf_counter() - t0
     t0 = time.perf_counter()
-    ops = build_products(full, boundaries, dx, bases)
+    ops = build_products(full, boundaries, dx, bases, iop=obstacle is not None)
     t["products_s"] = time.perf_counter() - t0
-    data = PlumeData(tuple(full), snaps, bases, ops, timings=t)
+    data = PlumeData(tuple(full), snaps, bases, ops, timings=t, box=start_box)
     # cubature
     t0 = time.perf_counter()
     q_pre, q_post = training_pairs(snaps, bases, cubature_snapshots)

@@ -5,6 +5,8 @@ Reference (aadjones/CFS_PLUME) -> here:
   FLUID_3D_MIC constants (FLUID_3D_MIC.cpp:30-56)            default_params
   addSmokeColumn (FLUID_3D_MIC.cpp:655-703)                 PlumeSolver.add_smoke (cfs_plume_add_smoke)
   stepPlume (SUBSPACE_FLUID_3D_COMPRESSED_EIGEN.cpp:453-501) PlumeSolver.step (cfs_plume_step)
+  initOutOfCoreIOP (:145-200)                               PlumeSolver.set_obstacle (cfs_plume_set_obstacle)
+  stepMovingObstacle (:508-566)                             PlumeSolver.step_obstacle (cfs_plume_step_obstacle)
 
 A step is kernel launches on one stream (forces, project, density and heat
 advection, subspace advection, reduced diffusion and pressure projection,
@@ -132,6 +134,38 @@ class PlumeSolver:
         (reducedCompressed3D.cpp:385-386)."""
         self.add_smoke(density, heat, stream)
         self.step(velocity, density, heat, q, stream)
+
+    def set_obstacle(self, U_iop: DeviceArray, U_proj: DeviceArray, iop_T_proj: np.ndarray, ptof_iop: np.ndarray,
+                     inverse_iop: np.ndarray):
+        """The moving-obstacle matrices: U_iop (r_iop, 3 Npeel) and U_preproject
+        (r_proj, 3 Npeel) on the device (kept alive here), U_iop^T U_preproject
+        (r_iop x r_proj), and ptof / inverseProduct of the IOP basis (r_fin x r_iop)."""
+        r_iop = U_iop.shape[0]
+        mats = [np.ascontiguousarray(m, np.float64) for m in (iop_T_proj, ptof_iop, inverse_iop)]
+        if U_proj.shape[0] != self.r_proj or U_iop.shape[1] != U_proj.shape[1]:
+            raise ValueError("U_preproject must have r_proj modes on the plume's grid")
+        if mats[0].shape != (r_iop, self.r_proj):
+            raise ValueError(f"U_iop^T U_preproject must be {r_iop} x {self.r_proj}")
+        for m in mats[1:]:
+            if m.shape != (self.r_fin, r_iop):
+                raise ValueError(f"ptof and inverseProduct of the IOP basis must be {self.r_fin} x {r_iop}")
+        _lib.check(_lib.lib().cfs_plume_set_obstacle(self._h, r_iop, U_iop.ptr, U_proj.ptr, mats[0].ctypes.data,
+                                                     mats[1].ctypes.data, mats[2].ctypes.data),
+                   "cfs_plume_set_obstacle")
+        self.r_iop = r_iop
+        self._obstacle = (U_iop, U_proj)
+
+    def step_obstacle(self, box, velocity: DeviceArray, density: DeviceArray, heat: DeviceArray, q: DeviceArray,
+                      stream: Optional[Stream] = None):
+        """stepMovingObstacle with the box at its current position."""
+        _lib.check(_lib.lib().cfs_plume_step_obstacle(self._h, box.ref, velocity.ptr, density.ptr, heat.ptr, q.ptr,
+                                                      _sptr(stream)), "stepMovingObstacle")
+
+    def frame_obstacle(self, box, velocity, density, heat, q, stream: Optional[Stream] = None):
+        """addSmokeColumn, stepMovingObstacle, then the box advanced."""
+        self.add_smoke(density, heat, stream)
+        self.step_obstacle(box, velocity, density, heat, q, stream)
+        box.advance()
 
     def capture(self, velocity, density, heat, q, stream: Stream) -> Graph:
         """A HIP graph of one frame on these arrays."""

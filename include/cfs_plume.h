@@ -398,6 +398,65 @@ int cfs_fluid_step(cfs_fluid* f, const cfs_fluid_snapshots* snapshots, void* str
 int cfs_fluid_last_solve(const cfs_fluid* f, int* iterations, double* residual_max);
 int cfs_fluid_destroy(cfs_fluid* f);
 
+/* ---- moving box obstacle ----------------------------------------------
+ * BOX (src/geometry/BOX.h, BOX.cpp:21-131): a box in the unit cube spinning
+ * about the z axis with `period` and sliding its centre along x as
+ * 0.25 sin(phi), phi = 2 pi t / translation_period.  A cell (full-grid index
+ * x, y, z) is inside when its centre (x + 0.5, y + 0.5, z + 0.5) / (X, Y, Z)
+ * (FLUID_3D_MIC::cellCenter :1704-1733), rotated back by theta about the
+ * centre, lies within half_lengths; it then moves with omega x r + the slide
+ * velocity (update_r, update_linearVelocity). */
+typedef struct {
+  double center[3];
+  double original_center[3];
+  double half_lengths[3];
+  double period, translation_period;
+  double theta, phi, time, dt;
+} cfs_box;
+/* BOX::BOX(center, lengths, period, translationPeriod) + set_dt (BOX.cpp:21-37). */
+int cfs_box_init(const double center[3], const double lengths[3], double period, double translation_period,
+                 double dt, cfs_box* box);
+/* translate_center, spin, update_time (BOX.cpp:105-126, BOX.h:39-43): the
+ * per-step advance of SUBSPACE_FLUID_3D_EIGEN.cpp:1715-1720. */
+int cfs_box_advance(cfs_box* box);
+/* FLUID_3D_MIC::setInitialVelocity (:744-763): the interior cells inside the
+ * box take its velocity. */
+int cfs_fluid_set_initial_velocity(cfs_fluid* f, const cfs_box* box, void* stream);
+/* FLUID_3D_MIC::stepWithMovingObstacle (:236-310): the step without
+ * buoyancy, with the box's cells stomped to its velocity
+ * (setPeeledSparseMovingIOP :1645-1702) between the damping and the
+ * projection.  iop_snapshot (device, 3 (X-2)(Y-2)(Z-2) doubles, may be null)
+ * receives the peeled post-stomp velocity (appendStreams' IOP stream). */
+int cfs_fluid_step_obstacle(cfs_fluid* f, const cfs_box* box, const cfs_fluid_snapshots* snapshots,
+                            double* iop_snapshot, void* stream);
+/* reducedSetMovingBox (SUBSPACE_FLUID_3D_COMPRESSED_EIGEN.cpp:779-790) with
+ * the pair of buildObstacleMatrices (SUBSPACE_FLUID_3D_EIGEN.cpp:1699-1711):
+ * q_out = R q + n, R = U_iop^T U_proj - U_iop^T C U_proj, n = U_iop^T nvec,
+ * evaluated without R as iop_T_proj q + sum over the box's cells of
+ * U_iop[c]^T (v_box(c) - U_proj[c] q).  U_iop, U_proj: device FP64 dense
+ * bases, 3 X Y Z rows (peeled grid X x Y x Z), column-major; iop_T_proj:
+ * device r_iop x r_proj row-major (U_iop^T U_proj); q (r_proj), q_out
+ * (r_iop) device.  scratch: cfs_reduced_iop_scratch_bytes(X, Y, Z) bytes of
+ * device memory, or null (allocated on the stream for the call). */
+int64_t cfs_reduced_iop_scratch_bytes(int X, int Y, int Z);
+int cfs_reduced_iop(const double* U_iop, int r_iop, const double* U_proj, int r_proj, const double* iop_T_proj,
+                    int X, int Y, int Z, const cfs_box* box, const double* q, double* q_out, void* scratch,
+                    void* stream);
+/* The moving-obstacle pipeline of the reduced plume (initOutOfCoreIOP and
+ * buildOutOfCoreMatricesIOP, SUBSPACE_FLUID_3D_COMPRESSED_EIGEN.cpp:145-200,
+ * SUBSPACE_FLUID_3D_EIGEN.cpp:1494-1659): U_iop and U_preproject (device,
+ * 3 Npeel x r dense FP64 column-major, kept by the caller), U_iop^T
+ * U_preproject (r_iop x r_proj) and the IOP basis' projection matrices
+ * projected.ptof = U_final^T U_iop and inverseProduct (r_fin x r_iop each),
+ * host row-major, copied. */
+int cfs_plume_set_obstacle(cfs_plume_solver* p, int r_iop, const double* U_iop, const double* U_preproject,
+                           const double* iop_T_preproject, const double* ptof_iop, const double* inverse_iop);
+/* stepMovingObstacle (SUBSPACE_FLUID_3D_COMPRESSED_EIGEN.cpp:508-566): as
+ * cfs_plume_step without buoyancy, with q = R q + n of the box at this step
+ * (cfs_reduced_iop) between the diffusion and the pressure projection. */
+int cfs_plume_step_obstacle(cfs_plume_solver* p, const cfs_box* box, float* velocity, float* density, float* heat,
+                            double* q, void* stream);
+
 /* ---- the basis builder's products (device FP64, column-major) ---------- */
 /* C = A^T B (m x n, row-major, device): A rows x m (leading dimension lda),
  * B rows x n (ldb).  The snapshot Gram matrices of the SVD
