@@ -197,7 +197,8 @@ def magnitudes(post: np.ndarray) -> np.ndarray:
     """1.0f / sqrtf(|b_s|^2) in single precision, 1 when |b_s|^2 <= 1e-7
     (generateImportanceSampledCubature :588-603)."""
     m = np.einsum("sr,sr->s", post, post)
-    inv = (np.float32(1.0) / np.sqrt(m.astype(np.float32))).astype(np.float64)
+    with np.errstate(divide="ignore"):  # zero samples (a fluid at rest) take 1 below
+        inv = (np.float32(1.0) / np.sqrt(m.astype(np.float32))).astype(np.float64)
     return np.where(np.abs(m) > 1e-7, inv, 1.0)
 
 
