@@ -251,18 +251,25 @@ __device__ __forceinline__ void step_body(const DevBasis& B, const float* __rest
     uint32_t li = l0;
     float open_sum = 0.0f;  // project sum of segment cur's tokens in earlier chunks (wave-uniform)
     int cur = 0;            // first segment not yet closed
-    // chunks alternate between the parser's two token buffers, so the
-    // loads of chunk c+2 are not waited for until chunk c+2
-    auto chunk = [&](const int base, auto par) {
+    // Each chunk runs in three stages: parse (tokens -> values and
+    // positions), compute (table reads, products, unproject atomics, long
+    // runs, the project prefix sums) and close (the project dots of the
+    // segments that end in the chunk).  The next chunk is parsed between a
+    // chunk's compute and its close, so the parse's arithmetic covers the
+    // latency of the atomics and prefix writes that the close waits for.
+    // Chunks alternate between the parser's two token buffers, so the loads
+    // of chunk c+2 are not waited for until chunk c+2.
+    auto parse = [&](const int base, auto par, Slots<NT>& sl) {
       constexpr int PAR = decltype(par)::value;
-      Slots<NT> sl;
       if (parser.plain(base))
         parser.template parse_slots<true, PAR>(base, sl);
       else
         parser.template parse_slots<false, PAR>(base, sl);
-      // {bits(1/s), end} of the segments that may close in this chunk
-      int2 se = make_int2(0, NO_END);
-      if (PROJ) se = *reinterpret_cast<const int2*>(&w.prm[min(cur + lane, r)].z);
+    };
+    // {bits(1/s), end} of the segments cur, cur+1, ... (lane j: cur + j)
+    int2 se = make_int2(0, NO_END);
+    // Returns the chunk's project sum (wave-uniform).
+    auto compute = [&](const int base, const Slots<NT>& sl) -> float {
       // Phases: all table reads first, then the arithmetic, then the atomics,
       // so that the LDS round trips of the NT slots overlap.
       float acc = 0.0f, pre[NT];
@@ -324,40 +331,66 @@ __device__ __forceinline__ void step_body(const DevBasis& B, const float* __rest
         li++;
         next_lr = li < l1 ? (int)B.lr[li] : NO_END;
       }
-      if (PROJ) {
-        const float incl = wave_scan_addf(acc);
-        const float excl = incl - acc;
-        float4* pf = reinterpret_cast<float4*>(w.pfx + NT * lane);
+      if (!PROJ) return 0.0f;
+      const float incl = wave_scan_addf(acc);
+      const float excl = incl - acc;
+      float4* pf = reinterpret_cast<float4*>(w.pfx + NT * lane);
 #pragma unroll
-        for (int q = 0; q < NT / 4; q++)
-          pf[q] = make_float4(excl + pre[4 * q], excl + pre[4 * q + 1], excl + pre[4 * q + 2], excl + pre[4 * q + 3]);
-        wave_lds_sync();
-        // segments cur, cur+1, ... whose last token lies in this chunk: the
-        // dot of segment cur + j is the chunk-local prefix at its last token
-        // minus the one at the last token of segment cur + j - 1 (lane j - 1,
-        // or bv: the last segment closed before, or -open_sum for segment cur)
-        float bv = -open_sum;
-        for (;;) {
-          const int c = cur + lane;
-          const bool closes = se.y <= base + CH;
-          const int pi = closes ? se.y - 1 - base : 0;
-          const float pv = w.pfx[pi];
-          const float q0 = closes ? w.qacc[c] : 0.0f;
-          const float prevv = __uint_as_float(wave_shr1(__float_as_uint(pv), __float_as_uint(bv)));
-          if (closes) w.qacc[c] = q0 + (pv - prevv) * __int_as_float(se.x);
-          const int n = __popcll(__ballot(closes));
-          if (n > 0) bv = readlanef(pv, n - 1);
-          cur += n;
-          if (n < WAVE) break;
-          se = *reinterpret_cast<const int2*>(&w.prm[min(cur + lane, r)].z);
-        }
-        open_sum = readlanef(incl, WAVE - 1) - bv;
-        wave_lds_sync();
-      }
+      for (int q = 0; q < NT / 4; q++)
+        pf[q] = make_float4(excl + pre[4 * q], excl + pre[4 * q + 1], excl + pre[4 * q + 2], excl + pre[4 * q + 3]);
+      se = *reinterpret_cast<const int2*>(&w.prm[min(cur + lane, r)].z);
+      return readlanef(incl, WAVE - 1);
     };
-    for (int base = 0; base < len; base += 2 * CH) {
-      chunk(base, std::integral_constant<int, 0>());
-      if (base + CH < len) chunk(base + CH, std::integral_constant<int, 1>());
+    auto close = [&](const int base, const float chunk_sum) {
+      if (!PROJ) return;
+      wave_lds_sync();
+      // segments cur, cur+1, ... whose last token lies in this chunk: the
+      // dot of segment cur + j is the chunk-local prefix at its last token
+      // minus the one at the last token of segment cur + j - 1 (lane j - 1,
+      // or bv: the last segment closed before, or -open_sum for segment cur)
+      float bv = -open_sum;
+      for (;;) {
+        const int c = cur + lane;
+        const bool closes = se.y <= base + CH;
+        const int pi = closes ? se.y - 1 - base : 0;
+        const float pv = w.pfx[pi];
+        const float q0 = closes ? w.qacc[c] : 0.0f;
+        const float prevv = __uint_as_float(wave_shr1(__float_as_uint(pv), __float_as_uint(bv)));
+        if (closes) w.qacc[c] = q0 + (pv - prevv) * __int_as_float(se.x);
+        const int n = __popcll(__ballot(closes));
+        if (n > 0) bv = readlanef(pv, n - 1);
+        cur += n;
+        if (n < WAVE) break;
+        se = *reinterpret_cast<const int2*>(&w.prm[min(cur + lane, r)].z);
+      }
+      open_sum = chunk_sum - bv;
+      wave_lds_sync();
+    };
+    // Pairs of chunks in the loop while a third chunk follows them, the last
+    // one or two after it.  Every path into the loop's head has issued the
+    // loads of both buffers in the same order (second buffer, then first),
+    // so each parse waits for its own loads only: a conditional parse inside
+    // the loop would make the wait-count pass wait for every load there.
+    Slots<NT> sa, sb;
+    parse(0, std::integral_constant<int, 0>(), sa);
+    int base = 0;
+    for (; base + 2 * CH < len; base += 2 * CH) {
+      float cs = compute(base, sa);
+      parser.template parse_slots<true, 1>(base + CH, sb);  // base + 2 CH < len: a whole chunk
+      close(base, cs);
+      cs = compute(base + CH, sb);
+      parse(base + 2 * CH, std::integral_constant<int, 0>(), sa);
+      close(base + CH, cs);
+    }
+    {
+      float cs = compute(base, sa);
+      const bool two = base + CH < len;
+      if (two) parse(base + CH, std::integral_constant<int, 1>(), sb);
+      close(base, cs);
+      if (two) {
+        cs = compute(base + CH, sb);
+        close(base + CH, cs);
+      }
     }
     complete = complete && parser.complete(r);
     // the next item's descriptor, segment parameters and block of v, in
@@ -392,10 +425,11 @@ step_kernel(DevBasis B, const float* __restrict__ v_in, const double* __restrict
 // Several independent bases in one persistent launch: workgroups
 // [wg_begin, wg_begin + wg_count) of the grid step entry e (entries sorted by
 // wg_begin, contiguous).
-// WPW waves per workgroup (a tile shape of the batch autotuner); 12 / WPW
-// workgroups per CU keep three waves per SIMD within the register budget.
+// WPW waves per workgroup (a tile shape of the batch autotuner).  The
+// second launch bound is HIP's minimum number of waves per SIMD: three,
+// the register budget of the step body (168 VGPRs), for either shape.
 template <int MODE, int WPW>
-__global__ void __launch_bounds__(WPW * WAVE, 12 / WPW)
+__global__ void __launch_bounds__(WPW * WAVE, 3)
 batch_step_kernel(const BatchEntry* __restrict__ E, int n) {
   int e = 0;
   while (e + 1 < n && (int)blockIdx.x >= E[e + 1].wg_begin) e++;

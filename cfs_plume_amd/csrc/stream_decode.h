@@ -275,17 +275,27 @@ class MarkedStream {
   __device__ MarkedStream(const int32_t* __restrict__ tok, const uint8_t* __restrict__ marks, int len)
       : tok_(tok), marks_(marks), len_(len) {
     lane_ = lane_id();
+    mark_shift_ = 8u * ((uint32_t)lane_ & 3u);
+    // first chunk's marks and tokens, then the second's: the first chunk
+    // waits only for its own loads
+    m_[0] = mark_word(marks_);
 #pragma unroll
-    for (int q = 0; q < NV; q++) {
-      buf_[0][q] = *reinterpret_cast<const int4*>(tok_ + NT * lane_ + 4 * q);
-      buf_[1][q] = *reinterpret_cast<const int4*>(tok_ + CH + NT * lane_ + 4 * q);
-    }
-    m_[0] = marks_[lane_];
-    m_[1] = marks_[CH / 8 + lane_];
+    for (int q = 0; q < NV; q++) buf_[0][q] = *reinterpret_cast<const int4*>(tok_ + NT * lane_ + 4 * q);
+    __builtin_amdgcn_sched_barrier(0);
+    m_[1] = mark_word(marks_ + CH / 8);
+#pragma unroll
+    for (int q = 0; q < NV; q++) buf_[1][q] = *reinterpret_cast<const int4*>(tok_ + CH + NT * lane_ + 4 * q);
   }
-
   // True if every token of the chunk at `base` belongs to the stream.
   __device__ __forceinline__ bool plain(int base) const { return base + CH <= len_; }
+
+  // The 32-bit word of a chunk's marks (64 bytes, one per lane) that holds
+  // this lane's byte.  Loaded as a word, the marks need no zero-extension:
+  // extending a byte load let the compiler widen the marks at the end of
+  // the previous chunk and wait for their load there.
+  __device__ __forceinline__ uint32_t mark_word(const uint8_t* __restrict__ chunk_marks) const {
+    return reinterpret_cast<const uint32_t*>(chunk_marks)[(uint32_t)lane_ >> 2];
+  }
 
   // PAR: the buffer holding this chunk (chunks alternate, starting with 0).
   template <bool PLAIN, int PAR>
@@ -298,15 +308,11 @@ class MarkedStream {
       t[4 * q + 2] = buf_[PAR][q].z;
       t[4 * q + 3] = buf_[PAR][q].w;
     }
-    const uint32_t m = m_[PAR];
+    // this lane's byte of the marks word (the bits above it are not read)
+    const uint32_t m = m_[PAR] >> mark_shift_;
     // the token before this lane's first (lane 0: the previous chunk's last)
     const int tm1 = (int)wave_shr1((uint32_t)t[NT - 1], (uint32_t)prev_last_);
     prev_last_ = (int)readlane((uint32_t)t[NT - 1], WAVE - 1);
-    // chunk base + 2*CH: a wave-uniform base address plus 32-bit lane offsets
-    const int32_t* tk = tok_ + (base + 2 * CH);
-#pragma unroll
-    for (int q = 0; q < NV; q++) buf_[PAR][q] = *reinterpret_cast<const int4*>(tk + (uint32_t)(NT * lane_ + 4 * q));
-    m_[PAR] = (marks_ + (base + 2 * CH) / 8)[(uint32_t)lane_];
     const int tid0 = base + NT * lane_;
 
     // Counts are selected biased by +2 (VALUE/DUP tokens 3, LEN tokens L,
@@ -316,9 +322,11 @@ class MarkedStream {
 #pragma unroll
     for (int i = 0; i < NT; i++) {
       const int prev = i == 0 ? tm1 : t[i - 1];
-      const bool len_tok = __builtin_amdgcn_ubfe(m, i, 1) != 0u;
-      int cnt2 = len_tok ? t[i] : 3;
-      int val = len_tok ? (t[i] > 2 ? prev : 0) : t[i];
+      // all ones for a length token: the two selects below are bitfield
+      // inserts with this mask, with no compare to make a lane mask
+      const int msk = __builtin_amdgcn_sbfe((int)m, i, 1);
+      int cnt2 = bit_select(msk, t[i], 3);
+      int val = bit_select(msk, t[i] > 2 ? prev : 0, t[i]);
       if (!PLAIN && tid0 + i >= len_) {
         cnt2 = 2;
         val = 0;
@@ -327,6 +335,16 @@ class MarkedStream {
       pre[i] = lane_sum;
       lane_sum += cnt2;
     }
+    // chunk base + 2*CH into this buffer, issued once its tokens are no
+    // longer read so that the loads may reuse their registers (a wave-uniform
+    // base address plus 32-bit lane offsets)
+    // The marks' load goes first: the compiler widens them at the end of the
+    // previous chunk, and a wait for them there then need not also wait for
+    // the tokens (the memory counter is in order).
+    m_[PAR] = mark_word(marks_ + (base + 2 * CH) / 8);
+    const int32_t* tk = tok_ + (base + 2 * CH);
+#pragma unroll
+    for (int q = 0; q < NV; q++) buf_[PAR][q] = *reinterpret_cast<const int4*>(tk + (uint32_t)(NT * lane_ + 4 * q));
     const int incl = wave_scan_add(lane_sum);
     const int p0 = pos_ + incl - lane_sum - 2 * NT * lane_;
     pos_ += readlane(incl, WAVE - 1) - 2 * NT * WAVE;
@@ -343,7 +361,8 @@ class MarkedStream {
   int len_;
   int lane_;
   int4 buf_[2][NV];  // tokens of the chunks at base and base + CH
-  uint32_t m_[2];
+  uint32_t m_[2];         // marks words of the chunks in the two buffers
+  uint32_t mark_shift_;   // 8 * (lane % 4): this lane's byte in its marks word
   int prev_last_ = 0;
   int pos_ = 0;
 };

@@ -129,6 +129,14 @@ __device__ __forceinline__ double wave_sum(double x) {
   return x;
 }
 
+// (a & mask) | (b & ~mask) as one v_bfi_b32.  Written as an instruction so
+// the optimiser does not turn a 0/-1 mask back into a compare and a select.
+__device__ __forceinline__ int bit_select(int mask, int a, int b) {
+  int r;
+  asm("v_bfi_b32 %0, %1, %2, %3" : "=v"(r) : "v"(mask), "v"(a), "v"(b));
+  return r;
+}
+
 // Orders this wave's LDS traffic (no cross-wave synchronisation).
 __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
