@@ -415,11 +415,24 @@ __device__ __forceinline__ void step_body(const DevBasis& B, const float* __rest
   }
 }
 
+// Workgroups are dispatched to the chip's NXCD dies in turn (workgroup i on
+// die i % NXCD).  Numbered die by die instead, neighbouring workgroups share
+// a die and its L2: the three components of a block, consecutive items that
+// neighbouring workgroups often split, then read the block's v rows once and
+// write its interleaved cells into the same cache lines.  A bijection of
+// [0, n) for any n.
+constexpr int NXCD = 8;
+__device__ __forceinline__ int die_major_index(int i, int n) {
+  const int x = i % NXCD, per = n / NXCD, extra = n % NXCD;
+  return x * per + min(x, extra) + i / NXCD;
+}
+
 template <int MODE>
 __global__ void __launch_bounds__(WAVES_PER_WG * WAVE, 3)  // <= 168 VGPRs: three waves per SIMD
 step_kernel(DevBasis B, const float* __restrict__ v_in, const double* __restrict__ q_in,
             float* __restrict__ field_out, const float* __restrict__ g_damp, const Tables* __restrict__ g_tab) {
-  step_body<MODE, WAVES_PER_WG>(B, v_in, q_in, field_out, g_damp, g_tab, B.partials, blockIdx.x, gridDim.x);
+  step_body<MODE, WAVES_PER_WG>(B, v_in, q_in, field_out, g_damp, g_tab, B.partials,
+                                die_major_index((int)blockIdx.x, (int)gridDim.x), gridDim.x);
 }
 
 // Several independent bases in one persistent launch: workgroups

@@ -271,11 +271,12 @@ class MarkedStream {
 
  public:
   // tok: 16-byte aligned block stream of `len` tokens followed by at least
-  // two chunks of readable tokens; marks: its LEN bitmap, likewise padded.
+  // one chunk of readable tokens; marks: its LEN bitmap, likewise padded.
   __device__ MarkedStream(const int32_t* __restrict__ tok, const uint8_t* __restrict__ marks, int len)
       : tok_(tok), marks_(marks), len_(len) {
     lane_ = lane_id();
     mark_shift_ = 8u * ((uint32_t)lane_ & 3u);
+    last_ = len > 0 ? (len - 1) / CH * CH : 0;
     // first chunk's marks and tokens, then the second's: the first chunk
     // waits only for its own loads
     m_[0] = mark_word(marks_);
@@ -341,8 +342,11 @@ class MarkedStream {
     // The marks' load goes first: the compiler widens them at the end of the
     // previous chunk, and a wait for them there then need not also wait for
     // the tokens (the memory counter is in order).
-    m_[PAR] = mark_word(marks_ + (base + 2 * CH) / 8);
-    const int32_t* tk = tok_ + (base + 2 * CH);
+    // chunk base + 2*CH, or the stream's last chunk again when that lies
+    // past the end (a cache hit instead of a read of tokens nobody uses)
+    const int nb = min(base + 2 * CH, last_);
+    m_[PAR] = mark_word(marks_ + nb / 8);
+    const int32_t* tk = tok_ + nb;
 #pragma unroll
     for (int q = 0; q < NV; q++) buf_[PAR][q] = *reinterpret_cast<const int4*>(tk + (uint32_t)(NT * lane_ + 4 * q));
     const int incl = wave_scan_add(lane_sum);
@@ -359,6 +363,7 @@ class MarkedStream {
   const int32_t* __restrict__ tok_;
   const uint8_t* __restrict__ marks_;
   int len_;
+  int last_;  // first token of the stream's last chunk
   int lane_;
   int4 buf_[2][NV];  // tokens of the chunks at base and base + CH
   uint32_t m_[2];         // marks words of the chunks in the two buffers

@@ -125,6 +125,30 @@ def runs(tmp_path_factory, gpu):
     return _pair(H.basis_from_ints(dims, 6, runs_zz, seed=13), tmp_path_factory, "runs")
 
 
+def longruns_zz(k, b, c):
+    """Nonzero runs of length 4..9 only, about 75 per segment, so a block
+    stream of 6 columns holds several hundred runs whose values past the
+    third are added from the loader's long-run list: the fused step reads
+    that list 64 runs at a time and refills it several times per stream."""
+    rng = np.random.default_rng(9000 + 100 * k + 10 * b + c)
+    out, n = [], 0
+    while n < 512:
+        L = int(min(512 - n, rng.integers(4, 10)))
+        v = int(rng.choice([-9, -4, -1, 1, 4, 9]))
+        if out and out[-1] == v:
+            v = -v  # keep runs maximal
+        out += [v] * L
+        n += L
+    return np.array(out, np.int32)
+
+
+@pytest.fixture(scope="module")
+def longruns(tmp_path_factory, gpu):
+    dims = (16, 8, 8)
+    print(f"[case] long nonzero runs dims={dims} r=6", flush=True)
+    return _pair(H.basis_from_ints(dims, 6, longruns_zz, seed=15), tmp_path_factory, "longruns")
+
+
 def wide_zz(k, b, c):
     """256 columns (the supported maximum): stretches of all-zero segments
     (3 tokens each, so more than 64 segments close inside one 512-token
@@ -168,7 +192,7 @@ def tiny(tmp_path_factory, gpu):
     return _pair(encode_matrix(U, dims).buffers(), tmp_path_factory, "tiny")
 
 
-CASES = ["smooth", "rough", "adversarial", "hazard", "runs", "wide", "ragged", "tiny"]
+CASES = ["smooth", "rough", "adversarial", "hazard", "runs", "longruns", "wide", "ragged", "tiny"]
 
 
 def test_hazards_are_listed(hazard):
