@@ -166,9 +166,31 @@ __device__ __forceinline__ void inverse_dct_block_mfma(float* __restrict__ field
   }
 }
 
+// Workgroups are dispatched to the chip's NXCD dies in turn: workgroup i
+// runs on die i % NXCD, whose L2 it shares with the die's other workgroups.
+constexpr int NXCD = 8;
+
+// Position of a wave among the nwaves items of each round of the static
+// schedule (item = position + round * nwaves; items are blocks in
+// decreasing-work order, three consecutive items per block).  DIE_GROUPED:
+// a block's three component items go to three consecutive waves of one die,
+// so they read its v rows and write its interleaved cells through one L2,
+// and consecutive blocks go to different dies, so every die gets an equal
+// share of every round (also of a last, partial one).  This needs each die's
+// waves to be a multiple of three; otherwise the position is the wave's
+// global index.
+template <int WPW, bool DIE_GROUPED>
+__device__ __forceinline__ int round_position(int wg, int nwg, int wave) {
+  if (DIE_GROUPED && nwg % NXCD == 0 && (nwg / NXCD * WPW) % 3 == 0) {
+    const int local = (wg / NXCD) * WPW + wave;  // wave's index on its die
+    return (local / 3 * NXCD + wg % NXCD) * 3 + local % 3;
+  }
+  return wg * WPW + wave;
+}
+
 // The fused step of one basis, run by workgroup `wg` of `nwg` that share it;
 // WPW waves per workgroup.
-template <int MODE, int WPW>
+template <int MODE, int WPW, bool DIE_GROUPED = false>
 __device__ __forceinline__ void step_body(const DevBasis& B, const float* __restrict__ v_in,
                                           const double* __restrict__ q_in, float* __restrict__ field_out,
                                           const float* __restrict__ g_damp, const Tables* __restrict__ g_tab,
@@ -189,7 +211,7 @@ __device__ __forceinline__ void step_body(const DevBasis& B, const float* __rest
   const int r = B.r;
   const int total = 3 * B.nB;
   const int nwaves = nwg * WPW;
-  int item = wg * WPW + wave;
+  int item = round_position<WPW, DIE_GROUPED>(wg, nwg, wave);
 
   ItemPrefetch nx;
   if (item < total) prefetch_item<PROJ>(B, v_in, item, nx);
@@ -415,24 +437,11 @@ __device__ __forceinline__ void step_body(const DevBasis& B, const float* __rest
   }
 }
 
-// Workgroups are dispatched to the chip's NXCD dies in turn (workgroup i on
-// die i % NXCD).  Numbered die by die instead, neighbouring workgroups share
-// a die and its L2: the three components of a block, consecutive items that
-// neighbouring workgroups often split, then read the block's v rows once and
-// write its interleaved cells into the same cache lines.  A bijection of
-// [0, n) for any n.
-constexpr int NXCD = 8;
-__device__ __forceinline__ int die_major_index(int i, int n) {
-  const int x = i % NXCD, per = n / NXCD, extra = n % NXCD;
-  return x * per + min(x, extra) + i / NXCD;
-}
-
 template <int MODE>
 __global__ void __launch_bounds__(WAVES_PER_WG * WAVE, 3)  // <= 168 VGPRs: three waves per SIMD
 step_kernel(DevBasis B, const float* __restrict__ v_in, const double* __restrict__ q_in,
             float* __restrict__ field_out, const float* __restrict__ g_damp, const Tables* __restrict__ g_tab) {
-  step_body<MODE, WAVES_PER_WG>(B, v_in, q_in, field_out, g_damp, g_tab, B.partials,
-                                die_major_index((int)blockIdx.x, (int)gridDim.x), gridDim.x);
+  step_body<MODE, WAVES_PER_WG, true>(B, v_in, q_in, field_out, g_damp, g_tab, B.partials, blockIdx.x, gridDim.x);
 }
 
 // Several independent bases in one persistent launch: workgroups
