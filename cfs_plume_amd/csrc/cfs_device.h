@@ -59,7 +59,7 @@ struct DevBasis {
                               //   {base lo, base hi, len, k*nB+b}, {hz0, hz1, lr0, lr1}, {bits(blk_bound), b, k, first 64-byte chunk of len_bits}
   uint32_t* err;              // device error bits
   uint32_t* counter;          // [2] spare device words (kept zero)
-  double* partials;           // [grid][r] per-workgroup FP64 partial q
+  double* partials;           // [r][grid] per-workgroup FP64 partial q (column-major)
   int X, Y, Z;                // peeled dims
   int nbx, nby, nbz, nB;      // blocks per axis, total
   int r, nk;                  // columns, rows of the damping table (kmax - kmin + 1)
@@ -77,7 +77,7 @@ struct BatchEntry {
   double* q_out;         // q projected
   const float* damp;     // the basis's damping table
   const Tables* tab;     // the basis's small tables
-  double* partials;      // [wg_count][r] per-workgroup partial q
+  double* partials;      // [r][wg_count] per-workgroup partial q (column-major)
   int wg_begin, wg_count;
   int q_begin;           // first block of this entry in the batched reduce
   int pad;

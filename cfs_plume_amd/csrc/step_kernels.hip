@@ -426,13 +426,14 @@ __device__ __forceinline__ void step_body(const DevBasis& B, const float* __rest
   if (!complete && lane == 0) atomicOr(B.err, (uint32_t)ERR_INCOMPLETE);
 
   if (PROJ) {
-    // combine the waves' FP64 accumulators in a fixed order
+    // combine the waves' FP64 accumulators in a fixed order; column-major
+    // partials, so the reduction reads each column's row contiguously
     __syncthreads();
     for (int c = threadIdx.x; c < r; c += blockDim.x) {
       double s = 0.0;
       for (int wv = 0; wv < WPW; wv++)
         s += (double)carve_wave(lds + step_shared_lds_bytes(nk) + (size_t)wv * step_wave_lds_bytes(r), r).qacc[c];
-      partials[(size_t)wg * r + c] = s;
+      partials[(size_t)c * nwg + wg] = s;
     }
   }
 }
@@ -461,39 +462,32 @@ batch_step_kernel(const BatchEntry* __restrict__ E, int n) {
                        be.wg_count);
 }
 
-// q of every entry: block (e, c) sums partials[0..wg_count)[c] of entry e.
+// Sum of row[0..n) in a fixed order (FP64): per thread a strided sum, then
+// per wave a butterfly, then the four waves in order.  256 threads.
+__device__ __forceinline__ double block_row_sum(const double* __restrict__ row, int n) {
+  __shared__ double red[4];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) s += row[i];
+  s = wave_sum(s);
+  if (lane_id() == 0) red[threadIdx.x / WAVE] = s;
+  __syncthreads();
+  return (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+// q of every entry: block (e, c) sums row c of entry e's partials.
 __global__ void __launch_bounds__(256) batch_reduce_kernel(const BatchEntry* __restrict__ E, int n) {
-  __shared__ double red[256];
   int e = 0;
   while (e + 1 < n && (int)blockIdx.x >= E[e + 1].q_begin) e++;
   const int c = (int)blockIdx.x - E[e].q_begin;
-  const int r = E[e].B.r;
-  const double* part = E[e].partials;
-  double s = 0.0;
-  for (int i = threadIdx.x; i < E[e].wg_count; i += blockDim.x) s += part[(size_t)i * r + c];
-  red[threadIdx.x] = s;
-  __syncthreads();
-  for (int h = 128; h > 0; h >>= 1) {
-    if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) E[e].q_out[c] = red[0];
+  const double s = block_row_sum(E[e].partials + (size_t)c * E[e].wg_count, E[e].wg_count);
+  if (threadIdx.x == 0) E[e].q_out[c] = s;
 }
 
-// q[c] = sum over workgroups of partials[wg][c], in workgroup order (FP64).
-__global__ void __launch_bounds__(256) reduce_q_kernel(const double* __restrict__ partials, int nrows, int r,
+// q[c] = sum over workgroups of partials[c][wg], in a fixed order (FP64).
+__global__ void __launch_bounds__(256) reduce_q_kernel(const double* __restrict__ partials, int nrows,
                                                        double* __restrict__ q) {
-  __shared__ double red[256];
-  const int c = blockIdx.x;
-  double s = 0.0;
-  for (int i = threadIdx.x; i < nrows; i += blockDim.x) s += partials[(size_t)i * r + c];
-  red[threadIdx.x] = s;
-  __syncthreads();
-  for (int h = 128; h > 0; h >>= 1) {
-    if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) q[c] = red[0];
+  const double s = block_row_sum(partials + (size_t)blockIdx.x * nrows, nrows);
+  if (threadIdx.x == 0) q[blockIdx.x] = s;
 }
 
 }  // namespace cfs
@@ -529,7 +523,7 @@ hipError_t launch_step(const DevBasis& B, int mode, const float* v_in, const dou
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   if (mode & MODE_PROJECT) {
-    hipLaunchKernelGGL(reduce_q_kernel, dim3(B.r), dim3(256), 0, s, B.partials, grid, B.r, q_out);
+    hipLaunchKernelGGL(reduce_q_kernel, dim3(B.r), dim3(256), 0, s, B.partials, grid, q_out);
     e = hipGetLastError();
   }
   return e;
