@@ -53,6 +53,25 @@ __device__ __forceinline__ void dct8_fwd(const float a[BS], float X[BS]) {
   X[7] = DC7 * d0 - DC5 * d1 + DC3 * d2 - DC1 * d3;
 }
 
+// a = C^T X (DCT-III, the inverse of dct8_fwd) by the same butterflies.
+__device__ __forceinline__ void dct8_inv(const float X[BS], float a[BS]) {
+  const float E0 = DC4 * (X[0] + X[4]), E1 = DC4 * (X[0] - X[4]);
+  const float P = DC2 * X[2] + DC6 * X[6], Q = DC6 * X[2] - DC2 * X[6];
+  const float e0 = E0 + P, e3 = E0 - P, e1 = E1 + Q, e2 = E1 - Q;
+  const float o0 = DC1 * X[1] + DC3 * X[3] + DC5 * X[5] + DC7 * X[7];
+  const float o1 = DC3 * X[1] - DC7 * X[3] - DC1 * X[5] - DC5 * X[7];
+  const float o2 = DC5 * X[1] - DC1 * X[3] + DC7 * X[5] + DC3 * X[7];
+  const float o3 = DC7 * X[1] - DC5 * X[3] + DC3 * X[5] - DC1 * X[7];
+  a[0] = e0 + o0;
+  a[7] = e0 - o0;
+  a[1] = e1 + o1;
+  a[6] = e1 - o1;
+  a[2] = e2 + o2;
+  a[5] = e2 - o2;
+  a[3] = e3 + o3;
+  a[4] = e3 - o3;
+}
+
 // Offset (in floats) of component k of full-grid cell (gx+1, gy+1, gz+1)
 // of the interleaved field (VECTOR3_FIELD_3D.cpp:69-85).  Full grids here
 // hold fewer than 2^31 floats (the loader checks).

@@ -109,61 +109,66 @@ __device__ __forceinline__ void prefetch_item(const DevBasis& B, const float* __
 }
 
 // Inverse DCT of the zigzag-ordered fixed-point accumulator `az` (value =
-// az * unit) on MFMA (mfma_idct.h: one block in the first half of a pair) and
-// the store of the interior cells of component k into the full-grid field
-// (UnitaryBlockDCTEigen(-1) + AssimilateBlocksEigen + subfield +
-// setWithPeeled, COMPRESSION.cpp:2880-2898, VECTOR3_FIELD_3D.cpp:3415-3430).
-// Leaves az zero.
-__device__ __forceinline__ void inverse_dct_block_mfma(float* __restrict__ field, const DevBasis& B, int k, int b,
-                                                       const uint16_t* zig, int* az, float unit,
-                                                       const MfmaFrags& mf) {
+// az * unit) and the store of the interior cells of component k into the
+// full-grid field (UnitaryBlockDCTEigen(-1) + AssimilateBlocksEigen +
+// subfield + setWithPeeled, COMPRESSION.cpp:2880-2898,
+// VECTOR3_FIELD_3D.cpp:3415-3430).  Leaves az zero.
+//
+// Three FP32 butterfly passes (dct8_inv), one 8-point line per lane, with
+// two transposes through `scr` (576 floats: eight planes of 64 padded to 72,
+// so that each transpose read or write touches every bank at most twice;
+// the wave's vz, free once the item's chunks are done, and the first 64
+// entries of az, which follows it and is cleared last):
+//   x: lane (v, w) gathers u = 0..7, writes x = 0..7 to scr[72 w + 8 v + x]
+//   y: lane (x, w) reads v = 0..7, writes y = 0..7 to scr[72 y + 8 w + x]
+//   z: lane (x, y) reads w = 0..7 and stores the cells (x, y, z = 0..7), so
+//      each store instruction writes an 8 x 8 (x, y) plane of the block.
+__device__ __forceinline__ void inverse_dct_block(float* __restrict__ field, const DevBasis& B, int k, int b,
+                                                  const uint16_t* zig, int* az, float unit, float* scr) {
   const int lane = lane_id();
-  const int g = lane >> 4, n = lane & 15;
-  // lane (n, g < 2) holds row (w, v) = 16 t + n, u = 4 g .. 4 g + 3; lanes g >= 2 (second block) hold zeros
-  float4 in[4];
+  const int lo = lane & 7, hi = lane >> 3;
+  float X[BS], a[BS];
+  {
+    // lane (v, w) = (lo, hi): natural indices u + 8 v + 64 w, u = 0..7
+    const uint4 zq = *reinterpret_cast<const uint4*>(zig + BS * lane);
+    const uint32_t zw[4] = {zq.x, zq.y, zq.z, zq.w};
 #pragma unroll
-  for (int t = 0; t < 4; t++) {
-    float v[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-    if (g < 2) {
-      const int nat = (16 * t + n) * BS + 4 * g;
-#pragma unroll
-      for (int i = 0; i < 4; i++) v[i] = (float)az[zig[nat + i]] * unit;
-    }
-    in[t] = make_float4(v[0], v[1], v[2], v[3]);
+    for (int u = 0; u < BS; u++) X[u] = (float)az[(zw[u >> 1] >> (16 * (u & 1))) & 0xffffu] * unit;
+    dct8_inv(X, a);
+    wave_lds_sync();
+    float4* row = reinterpret_cast<float4*>(scr + 72 * hi + BS * lo);
+    row[0] = make_float4(a[0], a[1], a[2], a[3]);
+    row[1] = make_float4(a[4], a[5], a[6], a[7]);
   }
   wave_lds_sync();
-  for (int i = lane; i < BC; i += WAVE) az[i] = 0;
-  bf16x4_t b1h, b1l;
+  {
+    // lane (x, w) = (lo, hi)
 #pragma unroll
-  for (int i = 0; i < 4; i++) {
-    b1h[i] = (short)mf.s1b[0][lane][i];
-    b1l[i] = (short)mf.s1b[1][lane][i];
+    for (int v = 0; v < BS; v++) X[v] = scr[72 * hi + BS * v + lo];
+    dct8_inv(X, a);
+    wave_lds_sync();
+#pragma unroll
+    for (int y = 0; y < BS; y++) scr[72 * y + BS * hi + lo] = a[y];
   }
-  bf16x8_t b2h[2], b2l[2];
-  idct_x_pass(in, b1h, b1l, b2h, b2l);
-  // (y, z) pass tile by tile (one tile's operands live at a time); lane
-  // (n < 8, g) stores cells x = n, (z, y) = 16 t + 4 g + i
-  const int bx = b % B.nbx, by = (b / B.nbx) % B.nby, bz = b / (B.nbx * B.nby);
-  const int gx = bx * BS + n;
-  const bool store = n < BS && gx < B.X;
-#pragma unroll 1
-  for (int t = 0; t < 4; t++) {
-    bf16x8_t a2h[2], a2l[2];
+  wave_lds_sync();
+  {
+    // lane (x, y) = (lo, hi)
 #pragma unroll
-    for (int ks = 0; ks < 2; ks++) {
-      a2h[ks] = as_bf16x8(*reinterpret_cast<const int4*>(mf.s2a[0][t][ks][lane]));
-      a2l[ks] = as_bf16x8(*reinterpret_cast<const int4*>(mf.s2a[1][t][ks][lane]));
-    }
-    const f32x4_t d = idct_yz_tile(a2h, a2l, b2h, b2l);
-    if (store) {
+    for (int w = 0; w < BS; w++) X[w] = scr[72 * hi + BS * w + lo];
+    dct8_inv(X, a);
+    const int bx = b % B.nbx, by = (b / B.nbx) % B.nby, bz = b / (B.nbx * B.nby);
+    const int gx = bx * BS + lo, gy = by * BS + hi;
+    if (gx < B.X && gy < B.Y) {
+      float* cell = field + field_offset(B, gx, gy, bz * BS, k);
+      const int plane = 3 * (B.X + 2) * (B.Y + 2);
 #pragma unroll
-      for (int i = 0; i < 4; i++) {
-        const int q = 16 * t + 4 * g + i;
-        const int gy = by * BS + (q & 7), gz = bz * BS + (q >> 3);
-        if (gy < B.Y && gz < B.Z) field[field_offset(B, gx, gy, gz, k)] = d[i];
-      }
+      for (int z = 0; z < BS; z++)
+        if (bz * BS + z < B.Z) cell[z * plane] = a[z];
     }
   }
+  wave_lds_sync();
+  for (int i = lane; i < BC; i += WAVE) az[i] = 0;  // also the part of scr that spilled into az
+  wave_lds_sync();
 }
 
 // Workgroups are dispatched to the chip's NXCD dies in turn: workgroup i
@@ -203,7 +208,7 @@ __device__ __forceinline__ void step_body(const DevBasis& B, const float* __rest
   // static LDS: its addresses are compile-time constants, so dz4[j] reads
   // address j + constant with no address arithmetic
   __shared__ uint8_t dz4[BC];
-  __shared__ uint16_t zig[BC];  // used by the per-block DCTs
+  __shared__ __align__(16) uint16_t zig[BC];  // used by the per-block DCTs
   float* damp = reinterpret_cast<float*>(lds);
   const int wave = uniform((int)(threadIdx.x / WAVE));
   const int lane = lane_id();
@@ -420,7 +425,7 @@ __device__ __forceinline__ void step_body(const DevBasis& B, const float* __rest
     if (item + nwaves < total) prefetch_item<PROJ>(B, v_in, item + nwaves, nx);
     wave_lds_sync();
 
-    if (UNPROJ) inverse_dct_block_mfma(field_out, B, k, b, zig, w.az, inv_scale, g_tab->mf);
+    if (UNPROJ) inverse_dct_block(field_out, B, k, b, zig, w.az, inv_scale, w.vz);
     wave_lds_sync();
   }
   if (!complete && lane == 0) atomicOr(B.err, (uint32_t)ERR_INCOMPLETE);
