@@ -12,8 +12,8 @@
 //
 // Precision: tokens and positions are exact integers; dequantisation is
 // int * (damp^gamma * (1/s)) in FP32 (the reference evaluates it in FP64,
-// COMPRESSION.cpp:1029-1056); per-block dots are FP32, summed in FP32 over
-// the few blocks one wave handles and in FP64 across waves.
+// COMPRESSION.cpp:1029-1056); per-segment dots are FP32, summed in FP64 over
+// the blocks one wave handles and across waves.
 #include <hip/hip_runtime.h>
 
 #include <type_traits>
@@ -28,7 +28,9 @@ namespace cfs {
 
 enum : int { MODE_PROJECT = 1, MODE_UNPROJECT = 2, MODE_BOTH = 3 };
 
-constexpr int WAVES_PER_WG = 4;
+// The single-basis step runs one workgroup of STEP_WAVES waves per CU (three
+// per SIMD); the batched step keeps the autotuned 2- or 4-wave shapes.
+constexpr int STEP_WAVES = 12;
 constexpr int NT = STEP_TPL;
 constexpr int CH = 64 * NT;
 constexpr int QREG = 4;  // segment parameters per lane: columns lane + 64*m, r <= 256
@@ -38,21 +40,21 @@ constexpr int NO_END = 0x7fffffff;
 struct WaveLds {
   int4* prm;     // [r+1] {damp row byte offset, bits(fixed-point weight of q_c), bits(2^64/s), end offset};
                  // entry r is {0, 0, 0, NO_END} (tokens past the stream's end index it)
-  float* qacc;   // [r] this wave's project accumulator (the dots of its items)
+  double* qacc;  // [r] this wave's project accumulator (the dots of its items)
   float* vz;     // [512] DCT(v) block in zigzag order
   int* az;       // [512] unproject accumulator in zigzag order, fixed point
   float* pfx;    // [512] project prefix sums of the current chunk; DCT scratch
 };
 
 __host__ __device__ inline size_t step_wave_lds_bytes(int r) {
-  return (size_t)(r + 1) * 16 + 3 * (size_t)BC * 4 + ((size_t)r * 4 + 15) / 16 * 16;
+  return (size_t)(r + 1) * 16 + 3 * (size_t)BC * 4 + ((size_t)r * 8 + 15) / 16 * 16;
 }
 // Shared part of the workgroup's dynamic LDS: the damping rows (the tables
 // 4*(u+v+w) and zigzag by index are static LDS, 3*512 bytes).
 __host__ __device__ inline size_t step_shared_lds_bytes(int nk) {
   return ((size_t)nk * NDIAG * 4 + 15) / 16 * 16;
 }
-__host__ inline size_t step_lds_bytes(int r, int nk, int wpw = WAVES_PER_WG) {
+__host__ inline size_t step_lds_bytes(int r, int nk, int wpw) {
   return step_shared_lds_bytes(nk) + (size_t)wpw * step_wave_lds_bytes(r);
 }
 
@@ -63,7 +65,7 @@ __device__ __forceinline__ WaveLds carve_wave(char* base, int r) {
   w.vz = reinterpret_cast<float*>(p); p += BC * 4;
   w.az = reinterpret_cast<int*>(p); p += BC * 4;
   w.pfx = reinterpret_cast<float*>(p); p += BC * 4;
-  w.qacc = reinterpret_cast<float*>(p);
+  w.qacc = reinterpret_cast<double*>(p);
   return w;
 }
 
@@ -194,8 +196,13 @@ __device__ __forceinline__ int round_position(int wg, int nwg, int wave) {
 }
 
 // The fused step of one basis, run by workgroup `wg` of `nwg` that share it;
-// WPW waves per workgroup.
-template <int MODE, int WPW, bool DIE_GROUPED = false>
+// WPW waves per workgroup.  Item slots of the workgroup: slot s is round
+// s / WPW at position round_position(wg, nwg, s % WPW), i.e. the items a
+// static round-robin would give its waves, in round order.  DYNAMIC: each
+// wave starts with slot `wave` and then takes the workgroup's next slot from
+// an LDS counter, so the waves of a CU that the SIMD arbiter runs faster (the
+// older ones) take more items; otherwise wave w takes slots w, w + WPW, ...
+template <int MODE, int WPW, bool DIE_GROUPED = false, bool DYNAMIC = false>
 __device__ __forceinline__ void step_body(const DevBasis& B, const float* __restrict__ v_in,
                                           const double* __restrict__ q_in, float* __restrict__ field_out,
                                           const float* __restrict__ g_damp, const Tables* __restrict__ g_tab,
@@ -209,6 +216,7 @@ __device__ __forceinline__ void step_body(const DevBasis& B, const float* __rest
   // address j + constant with no address arithmetic
   __shared__ uint8_t dz4[BC];
   __shared__ __align__(16) uint16_t zig[BC];  // used by the per-block DCTs
+  __shared__ uint32_t next_slot;               // DYNAMIC: the workgroup's next unassigned slot
   float* damp = reinterpret_cast<float*>(lds);
   const int wave = uniform((int)(threadIdx.x / WAVE));
   const int lane = lane_id();
@@ -226,8 +234,9 @@ __device__ __forceinline__ void step_body(const DevBasis& B, const float* __rest
     zig[i] = g_tab->zig[i];
   }
   for (int i = lane; i < BC; i += WAVE) w.az[i] = 0;
-  for (int c = lane; c < r; c += WAVE) w.qacc[c] = 0.0f;
+  for (int c = lane; c < r; c += WAVE) w.qacc[c] = 0.0;
   if (lane == 0) w.prm[r] = make_int4(0, 0, 0, NO_END);
+  if (DYNAMIC && threadIdx.x == 0) next_slot = WPW;
   // q is the same for every block: keep this lane's columns in registers
   float qf[QREG];
   float qmax = 0.0f;
@@ -240,9 +249,9 @@ __device__ __forceinline__ void step_body(const DevBasis& B, const float* __rest
   if (UNPROJ) qmax = wave_max(qmax);
   __syncthreads();
   bool complete = true;
-  // Static round-robin over items sorted by decreasing work: the three
-  // components of a block go to neighbouring waves and run together.
-  for (; item < total; item += nwaves) {
+  // Items sorted by decreasing work: the three components of a block go to
+  // neighbouring waves and run together.
+  while (item < total) {
     const ItemPrefetch it = nx;
     const int b = (int)it.d2.y;
     const int k = (int)it.d2.z;
@@ -381,9 +390,9 @@ __device__ __forceinline__ void step_body(const DevBasis& B, const float* __rest
         const bool closes = se.y <= base + CH;
         const int pi = closes ? se.y - 1 - base : 0;
         const float pv = w.pfx[pi];
-        const float q0 = closes ? w.qacc[c] : 0.0f;
+        const double q0 = closes ? w.qacc[c] : 0.0;
         const float prevv = __uint_as_float(wave_shr1(__float_as_uint(pv), __float_as_uint(bv)));
-        if (closes) w.qacc[c] = q0 + (pv - prevv) * __int_as_float(se.x);
+        if (closes) w.qacc[c] = q0 + (double)((pv - prevv) * __int_as_float(se.x));
         const int n = __popcll(__ballot(closes));
         if (n > 0) bv = readlanef(pv, n - 1);
         cur += n;
@@ -420,13 +429,22 @@ __device__ __forceinline__ void step_body(const DevBasis& B, const float* __rest
       }
     }
     complete = complete && parser.complete(r);
-    // the next item's descriptor, segment parameters and block of v, in
-    // flight during this item's inverse DCT
-    if (item + nwaves < total) prefetch_item<PROJ>(B, v_in, item + nwaves, nx);
+    // the next item (slots map to items in increasing order, so the first
+    // slot past the end ends the wave); its descriptor, segment parameters
+    // and block of v are in flight during this item's inverse DCT
+    int nxt = item + nwaves;
+    if (DYNAMIC) {
+      uint32_t sl = 0;
+      if (lane == 0) sl = atomicAdd(&next_slot, 1u);
+      const int slot = (int)readlane(sl, 0);
+      nxt = slot / WPW * nwaves + round_position<WPW, DIE_GROUPED>(wg, nwg, slot % WPW);
+    }
+    if (nxt < total) prefetch_item<PROJ>(B, v_in, nxt, nx);
     wave_lds_sync();
 
     if (UNPROJ) inverse_dct_block(field_out, B, k, b, zig, w.az, inv_scale, w.vz);
     wave_lds_sync();
+    item = nxt;
   }
   if (!complete && lane == 0) atomicOr(B.err, (uint32_t)ERR_INCOMPLETE);
 
@@ -437,17 +455,18 @@ __device__ __forceinline__ void step_body(const DevBasis& B, const float* __rest
     for (int c = threadIdx.x; c < r; c += blockDim.x) {
       double s = 0.0;
       for (int wv = 0; wv < WPW; wv++)
-        s += (double)carve_wave(lds + step_shared_lds_bytes(nk) + (size_t)wv * step_wave_lds_bytes(r), r).qacc[c];
+        s += carve_wave(lds + step_shared_lds_bytes(nk) + (size_t)wv * step_wave_lds_bytes(r), r).qacc[c];
       partials[(size_t)c * nwg + wg] = s;
     }
   }
 }
 
 template <int MODE>
-__global__ void __launch_bounds__(WAVES_PER_WG * WAVE, 3)  // <= 168 VGPRs: three waves per SIMD
+__global__ void __launch_bounds__(STEP_WAVES * WAVE, 3)  // <= 168 VGPRs: three waves per SIMD
 step_kernel(DevBasis B, const float* __restrict__ v_in, const double* __restrict__ q_in,
             float* __restrict__ field_out, const float* __restrict__ g_damp, const Tables* __restrict__ g_tab) {
-  step_body<MODE, WAVES_PER_WG, true>(B, v_in, q_in, field_out, g_damp, g_tab, B.partials, blockIdx.x, gridDim.x);
+  step_body<MODE, STEP_WAVES, true, true>(B, v_in, q_in, field_out, g_damp, g_tab, B.partials, blockIdx.x,
+                                          gridDim.x);
 }
 
 // Several independent bases in one persistent launch: workgroups
@@ -509,19 +528,19 @@ int step_partial_rows(int grid) { return grid; }
 
 hipError_t launch_step(const DevBasis& B, int mode, const float* v_in, const double* q_in, double* q_out,
                        float* field_out, const float* d_damp, const void* d_tab, int grid, hipStream_t s) {
-  const size_t lds = step_lds_bytes(B.r, B.nk);
+  const size_t lds = step_lds_bytes(B.r, B.nk, STEP_WAVES);
   const Tables* tab = reinterpret_cast<const Tables*>(d_tab);
   switch (mode) {
     case MODE_PROJECT:
-      hipLaunchKernelGGL(step_kernel<MODE_PROJECT>, dim3(grid), dim3(WAVES_PER_WG * WAVE), lds, s, B, v_in, q_in,
+      hipLaunchKernelGGL(step_kernel<MODE_PROJECT>, dim3(grid), dim3(STEP_WAVES * WAVE), lds, s, B, v_in, q_in,
                          field_out, d_damp, tab);
       break;
     case MODE_UNPROJECT:
-      hipLaunchKernelGGL(step_kernel<MODE_UNPROJECT>, dim3(grid), dim3(WAVES_PER_WG * WAVE), lds, s, B, v_in, q_in,
+      hipLaunchKernelGGL(step_kernel<MODE_UNPROJECT>, dim3(grid), dim3(STEP_WAVES * WAVE), lds, s, B, v_in, q_in,
                          field_out, d_damp, tab);
       break;
     default:
-      hipLaunchKernelGGL(step_kernel<MODE_BOTH>, dim3(grid), dim3(WAVES_PER_WG * WAVE), lds, s, B, v_in, q_in,
+      hipLaunchKernelGGL(step_kernel<MODE_BOTH>, dim3(grid), dim3(STEP_WAVES * WAVE), lds, s, B, v_in, q_in,
                          field_out, d_damp, tab);
       break;
   }
@@ -535,17 +554,17 @@ hipError_t launch_step(const DevBasis& B, int mode, const float* v_in, const dou
 }
 
 hipError_t step_occupancy(int mode, int r, int nk, int* wg_per_cu) {
-  const size_t lds = step_lds_bytes(r, nk);
+  const size_t lds = step_lds_bytes(r, nk, STEP_WAVES);
   hipError_t e;
   switch (mode) {
     case MODE_PROJECT:
-      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(wg_per_cu, step_kernel<MODE_PROJECT>, WAVES_PER_WG * WAVE, lds);
+      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(wg_per_cu, step_kernel<MODE_PROJECT>, STEP_WAVES * WAVE, lds);
       break;
     case MODE_UNPROJECT:
-      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(wg_per_cu, step_kernel<MODE_UNPROJECT>, WAVES_PER_WG * WAVE, lds);
+      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(wg_per_cu, step_kernel<MODE_UNPROJECT>, STEP_WAVES * WAVE, lds);
       break;
     default:
-      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(wg_per_cu, step_kernel<MODE_BOTH>, WAVES_PER_WG * WAVE, lds);
+      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(wg_per_cu, step_kernel<MODE_BOTH>, STEP_WAVES * WAVE, lds);
       break;
   }
   return e;
