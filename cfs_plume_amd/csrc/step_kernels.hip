@@ -177,20 +177,21 @@ __device__ __forceinline__ void inverse_dct_block(float* __restrict__ field, con
 // runs on die i % NXCD, whose L2 it shares with the die's other workgroups.
 constexpr int NXCD = 8;
 
-// Position of a wave among the nwaves items of each round of the static
-// schedule (item = position + round * nwaves; items are blocks in
-// decreasing-work order, three consecutive items per block).  DIE_GROUPED:
-// a block's three component items go to three consecutive waves of one die,
-// so they read its v rows and write its interleaved cells through one L2,
-// and consecutive blocks go to different dies, so every die gets an equal
-// share of every round (also of a last, partial one).  This needs each die's
-// waves to be a multiple of three; otherwise the position is the wave's
-// global index.
+// Position of a wave among the nwaves items of each round of the schedule
+// (item = position + round * nwaves; items are blocks in decreasing-work
+// order, three consecutive items per block).  DIE_GROUPED (WPW a multiple of
+// three): a block's three component items go to three consecutive waves of
+// one workgroup, so they read its v rows and write its interleaved cells
+// through one L2; consecutive blocks go to different dies, and a die's
+// blocks go to its workgroups in turn (the workgroup's first triple of
+// waves, then its second, ...), so every die and every workgroup gets an
+// equal share of every round, also of a last, partial one.  Otherwise the
+// position is the wave's global index.
 template <int WPW, bool DIE_GROUPED>
 __device__ __forceinline__ int round_position(int wg, int nwg, int wave) {
-  if (DIE_GROUPED && nwg % NXCD == 0 && (nwg / NXCD * WPW) % 3 == 0) {
-    const int local = (wg / NXCD) * WPW + wave;  // wave's index on its die
-    return (local / 3 * NXCD + wg % NXCD) * 3 + local % 3;
+  if (DIE_GROUPED && WPW % 3 == 0 && nwg % NXCD == 0) {
+    const int triple = (wave / 3) * (nwg / NXCD) + wg / NXCD;  // block index on the die within the round
+    return (triple * NXCD + wg % NXCD) * 3 + wave % 3;
   }
   return wg * WPW + wave;
 }
