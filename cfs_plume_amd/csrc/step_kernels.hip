@@ -57,6 +57,16 @@ __host__ __device__ inline size_t step_shared_lds_bytes(int nk) {
 __host__ inline size_t step_lds_bytes(int r, int nk, int wpw) {
   return step_shared_lds_bytes(nk) + (size_t)wpw * step_wave_lds_bytes(r);
 }
+// gfx950: 160 KB of LDS per CU, all of it available to one workgroup; the
+// step's static LDS (the 4*(u+v+w) and zigzag tables, the slot counter)
+// takes less than 2 KB.
+constexpr size_t LDS_PER_CU = 160 * 1024;
+constexpr size_t STEP_STATIC_LDS = 2048;
+// The single-basis step copies the inverse DCT's constant operand table
+// (17 KB) into its LDS, after the waves' buffers, when the table fits there.
+__host__ inline bool step_mf_in_lds(int mode, int r, int nk) {
+  return (mode & 2) != 0 && step_lds_bytes(r, nk, STEP_WAVES) + sizeof(MfmaFrags) + STEP_STATIC_LDS <= LDS_PER_CU;
+}
 
 __device__ __forceinline__ WaveLds carve_wave(char* base, int r) {
   WaveLds w;
@@ -111,66 +121,63 @@ __device__ __forceinline__ void prefetch_item(const DevBasis& B, const float* __
 }
 
 // Inverse DCT of the zigzag-ordered fixed-point accumulator `az` (value =
-// az * unit) and the store of the interior cells of component k into the
-// full-grid field (UnitaryBlockDCTEigen(-1) + AssimilateBlocksEigen +
-// subfield + setWithPeeled, COMPRESSION.cpp:2880-2898,
-// VECTOR3_FIELD_3D.cpp:3415-3430).  Leaves az zero.
-//
-// Three FP32 butterfly passes (dct8_inv), one 8-point line per lane, with
-// two transposes through `scr` (576 floats: eight planes of 64 padded to 72,
-// so that each transpose read or write touches every bank at most twice;
-// the wave's vz, free once the item's chunks are done, and the first 64
-// entries of az, which follows it and is cleared last):
-//   x: lane (v, w) gathers u = 0..7, writes x = 0..7 to scr[72 w + 8 v + x]
-//   y: lane (x, w) reads v = 0..7, writes y = 0..7 to scr[72 y + 8 w + x]
-//   z: lane (x, y) reads w = 0..7 and stores the cells (x, y, z = 0..7), so
-//      each store instruction writes an 8 x 8 (x, y) plane of the block.
-__device__ __forceinline__ void inverse_dct_block(float* __restrict__ field, const DevBasis& B, int k, int b,
-                                                  const uint16_t* zig, int* az, float unit, float* scr) {
+// az * unit) on MFMA (mfma_idct.h: one block in the first half of a pair) and
+// the store of the interior cells of component k into the full-grid field
+// (UnitaryBlockDCTEigen(-1) + AssimilateBlocksEigen + subfield +
+// setWithPeeled, COMPRESSION.cpp:2880-2898, VECTOR3_FIELD_3D.cpp:3415-3430).
+// Leaves az zero.  `mf` is the constant operand table, in LDS for the
+// single-basis step (so that its reads do not queue behind the next item's
+// prefetch in the in-order vector-memory counter) or in global memory.
+__device__ __forceinline__ void inverse_dct_block_mfma(float* __restrict__ field, const DevBasis& B, int k, int b,
+                                                       const uint16_t* zig, int* az, float unit,
+                                                       const MfmaFrags& mf) {
   const int lane = lane_id();
-  const int lo = lane & 7, hi = lane >> 3;
-  float X[BS], a[BS];
-  {
-    // lane (v, w) = (lo, hi): natural indices u + 8 v + 64 w, u = 0..7
-    const uint4 zq = *reinterpret_cast<const uint4*>(zig + BS * lane);
-    const uint32_t zw[4] = {zq.x, zq.y, zq.z, zq.w};
+  const int g = lane >> 4, n = lane & 15;
+  // lane (n, g < 2) holds row (w, v) = 16 t + n, u = 4 g .. 4 g + 3; lanes g >= 2 (second block) hold zeros
+  float4 in[4];
 #pragma unroll
-    for (int u = 0; u < BS; u++) X[u] = (float)az[(zw[u >> 1] >> (16 * (u & 1))) & 0xffffu] * unit;
-    dct8_inv(X, a);
-    wave_lds_sync();
-    float4* row = reinterpret_cast<float4*>(scr + 72 * hi + BS * lo);
-    row[0] = make_float4(a[0], a[1], a[2], a[3]);
-    row[1] = make_float4(a[4], a[5], a[6], a[7]);
+  for (int t = 0; t < 4; t++) {
+    float v[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    if (g < 2) {
+      const int nat = (16 * t + n) * BS + 4 * g;
+#pragma unroll
+      for (int i = 0; i < 4; i++) v[i] = (float)az[zig[nat + i]] * unit;
+    }
+    in[t] = make_float4(v[0], v[1], v[2], v[3]);
   }
   wave_lds_sync();
-  {
-    // lane (x, w) = (lo, hi)
+  for (int i = lane; i < BC; i += WAVE) az[i] = 0;
+  bf16x4_t b1h, b1l;
 #pragma unroll
-    for (int v = 0; v < BS; v++) X[v] = scr[72 * hi + BS * v + lo];
-    dct8_inv(X, a);
-    wave_lds_sync();
-#pragma unroll
-    for (int y = 0; y < BS; y++) scr[72 * y + BS * hi + lo] = a[y];
+  for (int i = 0; i < 4; i++) {
+    b1h[i] = (short)mf.s1b[0][lane][i];
+    b1l[i] = (short)mf.s1b[1][lane][i];
   }
-  wave_lds_sync();
-  {
-    // lane (x, y) = (lo, hi)
+  bf16x8_t b2h[2], b2l[2];
+  idct_x_pass(in, b1h, b1l, b2h, b2l);
+  // (y, z) pass tile by tile (one tile's operands live at a time); lane
+  // (n < 8, g) stores cells x = n, (z, y) = 16 t + 4 g + i
+  const int bx = b % B.nbx, by = (b / B.nbx) % B.nby, bz = b / (B.nbx * B.nby);
+  const int gx = bx * BS + n;
+  const bool store = n < BS && gx < B.X;
+#pragma unroll 1
+  for (int t = 0; t < 4; t++) {
+    bf16x8_t a2h[2], a2l[2];
 #pragma unroll
-    for (int w = 0; w < BS; w++) X[w] = scr[72 * hi + BS * w + lo];
-    dct8_inv(X, a);
-    const int bx = b % B.nbx, by = (b / B.nbx) % B.nby, bz = b / (B.nbx * B.nby);
-    const int gx = bx * BS + lo, gy = by * BS + hi;
-    if (gx < B.X && gy < B.Y) {
-      float* cell = field + field_offset(B, gx, gy, bz * BS, k);
-      const int plane = 3 * (B.X + 2) * (B.Y + 2);
+    for (int ks = 0; ks < 2; ks++) {
+      a2h[ks] = as_bf16x8(*reinterpret_cast<const int4*>(mf.s2a[0][t][ks][lane]));
+      a2l[ks] = as_bf16x8(*reinterpret_cast<const int4*>(mf.s2a[1][t][ks][lane]));
+    }
+    const f32x4_t d = idct_yz_tile(a2h, a2l, b2h, b2l);
+    if (store) {
 #pragma unroll
-      for (int z = 0; z < BS; z++)
-        if (bz * BS + z < B.Z) cell[z * plane] = a[z];
+      for (int i = 0; i < 4; i++) {
+        const int q = 16 * t + 4 * g + i;
+        const int gy = by * BS + (q & 7), gz = bz * BS + (q >> 3);
+        if (gy < B.Y && gz < B.Z) field[field_offset(B, gx, gy, gz, k)] = d[i];
+      }
     }
   }
-  wave_lds_sync();
-  for (int i = lane; i < BC; i += WAVE) az[i] = 0;  // also the part of scr that spilled into az
-  wave_lds_sync();
 }
 
 // Workgroups are dispatched to the chip's NXCD dies in turn: workgroup i
@@ -203,7 +210,7 @@ __device__ __forceinline__ int round_position(int wg, int nwg, int wave) {
 // wave starts with slot `wave` and then takes the workgroup's next slot from
 // an LDS counter, so the waves of a CU that the SIMD arbiter runs faster (the
 // older ones) take more items; otherwise wave w takes slots w, w + WPW, ...
-template <int MODE, int WPW, bool DIE_GROUPED = false, bool DYNAMIC = false>
+template <int MODE, int WPW, bool DIE_GROUPED = false, bool DYNAMIC = false, bool MF_LDS = false>
 __device__ __forceinline__ void step_body(const DevBasis& B, const float* __restrict__ v_in,
                                           const double* __restrict__ q_in, float* __restrict__ field_out,
                                           const float* __restrict__ g_damp, const Tables* __restrict__ g_tab,
@@ -230,6 +237,14 @@ __device__ __forceinline__ void step_body(const DevBasis& B, const float* __rest
   ItemPrefetch nx;
   if (item < total) prefetch_item<PROJ>(B, v_in, item, nx);
   for (int i = threadIdx.x; i < nk * NDIAG; i += blockDim.x) damp[i] = g_damp[i];
+  const MfmaFrags* mf = &g_tab->mf;
+  if (MF_LDS) {
+    MfmaFrags* m = reinterpret_cast<MfmaFrags*>(lds + step_shared_lds_bytes(nk) + (size_t)WPW * step_wave_lds_bytes(r));
+    const int4* src = reinterpret_cast<const int4*>(&g_tab->mf);
+    int4* dst = reinterpret_cast<int4*>(m);
+    for (int i = threadIdx.x; i < (int)(sizeof(MfmaFrags) / 16); i += blockDim.x) dst[i] = src[i];
+    mf = m;
+  }
   for (int i = threadIdx.x; i < BC; i += blockDim.x) {
     dz4[i] = (uint8_t)(4 * g_tab->dz[i]);
     zig[i] = g_tab->zig[i];
@@ -443,7 +458,7 @@ __device__ __forceinline__ void step_body(const DevBasis& B, const float* __rest
     if (nxt < total) prefetch_item<PROJ>(B, v_in, nxt, nx);
     wave_lds_sync();
 
-    if (UNPROJ) inverse_dct_block(field_out, B, k, b, zig, w.az, inv_scale, w.vz);
+    if (UNPROJ) inverse_dct_block_mfma(field_out, B, k, b, zig, w.az, inv_scale, *mf);
     wave_lds_sync();
     item = nxt;
   }
@@ -462,12 +477,12 @@ __device__ __forceinline__ void step_body(const DevBasis& B, const float* __rest
   }
 }
 
-template <int MODE>
+template <int MODE, bool MF_LDS>
 __global__ void __launch_bounds__(STEP_WAVES * WAVE, 3)  // <= 168 VGPRs: three waves per SIMD
 step_kernel(DevBasis B, const float* __restrict__ v_in, const double* __restrict__ q_in,
             float* __restrict__ field_out, const float* __restrict__ g_damp, const Tables* __restrict__ g_tab) {
-  step_body<MODE, STEP_WAVES, true, true>(B, v_in, q_in, field_out, g_damp, g_tab, B.partials, blockIdx.x,
-                                          gridDim.x);
+  step_body<MODE, STEP_WAVES, true, true, MF_LDS>(B, v_in, q_in, field_out, g_damp, g_tab, B.partials, blockIdx.x,
+                                                  gridDim.x);
 }
 
 // Several independent bases in one persistent launch: workgroups
@@ -529,20 +544,30 @@ int step_partial_rows(int grid) { return grid; }
 
 hipError_t launch_step(const DevBasis& B, int mode, const float* v_in, const double* q_in, double* q_out,
                        float* field_out, const float* d_damp, const void* d_tab, int grid, hipStream_t s) {
-  const size_t lds = step_lds_bytes(B.r, B.nk, STEP_WAVES);
+  const bool mfl = step_mf_in_lds(mode, B.r, B.nk);
+  const size_t lds = step_lds_bytes(B.r, B.nk, STEP_WAVES) + (mfl ? sizeof(MfmaFrags) : 0);
   const Tables* tab = reinterpret_cast<const Tables*>(d_tab);
+  const dim3 block(STEP_WAVES * WAVE);
   switch (mode) {
     case MODE_PROJECT:
-      hipLaunchKernelGGL(step_kernel<MODE_PROJECT>, dim3(grid), dim3(STEP_WAVES * WAVE), lds, s, B, v_in, q_in,
-                         field_out, d_damp, tab);
+      hipLaunchKernelGGL((step_kernel<MODE_PROJECT, false>), dim3(grid), block, lds, s, B, v_in, q_in, field_out,
+                         d_damp, tab);
       break;
     case MODE_UNPROJECT:
-      hipLaunchKernelGGL(step_kernel<MODE_UNPROJECT>, dim3(grid), dim3(STEP_WAVES * WAVE), lds, s, B, v_in, q_in,
-                         field_out, d_damp, tab);
+      if (mfl)
+        hipLaunchKernelGGL((step_kernel<MODE_UNPROJECT, true>), dim3(grid), block, lds, s, B, v_in, q_in, field_out,
+                           d_damp, tab);
+      else
+        hipLaunchKernelGGL((step_kernel<MODE_UNPROJECT, false>), dim3(grid), block, lds, s, B, v_in, q_in, field_out,
+                           d_damp, tab);
       break;
     default:
-      hipLaunchKernelGGL(step_kernel<MODE_BOTH>, dim3(grid), dim3(STEP_WAVES * WAVE), lds, s, B, v_in, q_in,
-                         field_out, d_damp, tab);
+      if (mfl)
+        hipLaunchKernelGGL((step_kernel<MODE_BOTH, true>), dim3(grid), block, lds, s, B, v_in, q_in, field_out, d_damp,
+                           tab);
+      else
+        hipLaunchKernelGGL((step_kernel<MODE_BOTH, false>), dim3(grid), block, lds, s, B, v_in, q_in, field_out,
+                           d_damp, tab);
       break;
   }
   hipError_t e = hipGetLastError();
@@ -555,20 +580,19 @@ hipError_t launch_step(const DevBasis& B, int mode, const float* v_in, const dou
 }
 
 hipError_t step_occupancy(int mode, int r, int nk, int* wg_per_cu) {
-  const size_t lds = step_lds_bytes(r, nk, STEP_WAVES);
-  hipError_t e;
+  const bool mfl = step_mf_in_lds(mode, r, nk);
+  const size_t lds = step_lds_bytes(r, nk, STEP_WAVES) + (mfl ? sizeof(MfmaFrags) : 0);
+  const int threads = STEP_WAVES * WAVE;
   switch (mode) {
     case MODE_PROJECT:
-      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(wg_per_cu, step_kernel<MODE_PROJECT>, STEP_WAVES * WAVE, lds);
-      break;
+      return hipOccupancyMaxActiveBlocksPerMultiprocessor(wg_per_cu, step_kernel<MODE_PROJECT, false>, threads, lds);
     case MODE_UNPROJECT:
-      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(wg_per_cu, step_kernel<MODE_UNPROJECT>, STEP_WAVES * WAVE, lds);
-      break;
+      return mfl ? hipOccupancyMaxActiveBlocksPerMultiprocessor(wg_per_cu, step_kernel<MODE_UNPROJECT, true>, threads, lds)
+                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(wg_per_cu, step_kernel<MODE_UNPROJECT, false>, threads, lds);
     default:
-      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(wg_per_cu, step_kernel<MODE_BOTH>, STEP_WAVES * WAVE, lds);
-      break;
+      return mfl ? hipOccupancyMaxActiveBlocksPerMultiprocessor(wg_per_cu, step_kernel<MODE_BOTH, true>, threads, lds)
+                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(wg_per_cu, step_kernel<MODE_BOTH, false>, threads, lds);
   }
-  return e;
 }
 
 template <int WPW>
