@@ -349,7 +349,11 @@ __device__ __forceinline__ void step_body(const DevBasis& B, const float* __rest
         }
         if (UNPROJ) {
 #pragma unroll
-          for (int i = 0; i < NT; i++) atomicAdd(&w.az[jj[i]], qv[i]);
+          // an atomic of zero changes nothing, so a zero term (the tokens of
+          // zero runs) goes to the lane's own entry instead of its position:
+          // in one slot the 64 lanes' positions lie about 8 apart and share
+          // few banks, entry `lane` shares none
+          for (int i = 0; i < NT; i++) atomicAdd(&w.az[qv[i] != 0 ? jj[i] : lane], qv[i]);
         }
       }
       // values P+3 .. P+L-1 of nonzero runs with L >= 4 (listed at load)
