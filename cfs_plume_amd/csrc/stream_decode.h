@@ -319,7 +319,7 @@ class MarkedStream {
     // Counts are selected biased by +2 (VALUE/DUP tokens 3, LEN tokens L,
     // tokens past the stream 2), which saves a subtraction per token; the
     // bias is removed once per lane and folded into the per-slot constant.
-    int lane_sum = 0, pre[NT];
+    int cnt[NT];
 #pragma unroll
     for (int i = 0; i < NT; i++) {
       const int prev = i == 0 ? tm1 : t[i - 1];
@@ -333,9 +333,11 @@ class MarkedStream {
         val = 0;
       }
       s.val[i] = val;
-      pre[i] = lane_sum;
-      lane_sum += cnt2;
+      cnt[i] = cnt2;
     }
+    // the lane's total as a tree of three-input adds (the positions follow
+    // the scan as a chain, one add per slot)
+    const int lane_sum = ((cnt[0] + cnt[1] + cnt[2]) + (cnt[3] + cnt[4] + cnt[5])) + (cnt[6] + cnt[7]);
     // chunk base + 2*CH into this buffer, issued once its tokens are no
     // longer read so that the loads may reuse their registers (a wave-uniform
     // base address plus 32-bit lane offsets)
@@ -352,8 +354,9 @@ class MarkedStream {
     const int incl = wave_scan_add(lane_sum);
     const int p0 = pos_ + incl - lane_sum - 2 * NT * lane_;
     pos_ += readlane(incl, WAVE - 1) - 2 * NT * WAVE;
+    s.pos[0] = p0;
 #pragma unroll
-    for (int i = 0; i < NT; i++) s.pos[i] = p0 + pre[i] - 2 * i;
+    for (int i = 1; i < NT; i++) s.pos[i] = s.pos[i - 1] + cnt[i - 1] - 2;
   }
 
   // Called after the last chunk: true if the stream produced nseg*512 values.
