@@ -873,7 +873,7 @@ def main():
                         f"percent={percent}, nBits=32",
             "grid": list(grid), "r": r, "blocks_per_component": basis.nB,
             "tokens": list(tokens), "bytes_per_step": step_bytes,
-            "accumulation": "fp32 per block and per wave, fp64 across workgroups",
+            "accumulation": "fp32 per segment dot, fp64 per wave and across workgroups",
             "parallelism": "single GPU",
         },
         "roofline": {

@@ -145,7 +145,7 @@ def test_c4_batch_graph_replay(batch16):
     """The whole batched step captured in a HIP graph: every replay equals the
     eager batched step bit for bit (with eager steps and synchronisations
     between replays); the batched fields equal each basis's own fused step bit
-    for bit, and its q agrees to the FP32 per-wave accumulation (the batch
+    for bit, and its q agrees to the FP64 summation order (the batch
     gives a basis fewer workgroups than its own launch, so its items are
     summed in other groupings); the largest and the smallest bases agree with
     the FP64 oracle."""
